@@ -1,0 +1,154 @@
+/*
+ * octopus_engine.h - C ABI of the MI355X soft-body FEM stepping engine.
+ *
+ * The entry points replace the reference's GPU dynamics path, which is a C++
+ * object graph driven by a script component (OctopusEngine, paths relative to
+ * octopus/):
+ *
+ *   Cuda_FEM_Dynamic::init            include/Script/Dynamic/Cuda_FEM_Dynamic.h:18-25
+ *   GPU_ParticleSystem(...)           include/GPU/GPU_ParticleSystem.h:56
+ *   GPU_VBD / GPU_VBD_FEM ctors       include/GPU/VBD/GPU_VBD.h:15, include/GPU/VBD/GPU_VBD_FEM.h:62-65
+ *   GPU_PBD / GPU_PBD_FEM ctors       include/GPU/PBD/GPU_PBD.h:37, include/GPU/PBD/GPU_PBD_FEM.h:19-20
+ *   GPU_FEM_Explicit ctor             include/GPU/Explicit/GPU_FEM_Explicit.h:14-15
+ *   GPU_Fix_Constraint / Box_Limit    include/GPU/GPU_Constraint.h:11-24
+ *   step(dt, substeps)                src/Script/Dynamic/Cuda_ParticleSystem_Dynamic.cpp:17-31
+ *   getters (positions, velocities..) include/GPU/GPU_ParticleSystem.h:58-65
+ *
+ * Conventions: plain pointers and sizes, float xyz arrays are 3*N interleaved,
+ * topology is int32 S*E (element node order of the reference converters).
+ * Every function returns an oe_status; on error oe_last_error() gives the
+ * message of the last failing call on this thread.  Input arrays are copied;
+ * the engine owns all device memory.  A scene may be used by one host thread
+ * at a time.  Device work is queued on the engine's stream; the read-back
+ * functions synchronise that stream.
+ */
+#ifndef OCTOPUS_ENGINE_H
+#define OCTOPUS_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t oe_status;
+enum {
+    OE_OK = 0,
+    OE_ERR_INVALID = 1,   /* bad argument / unsupported configuration */
+    OE_ERR_HIP = 2,       /* HIP runtime error */
+    OE_ERR_BUDGET = 3,    /* device memory budget exceeded */
+    OE_ERR_STATE = 4,     /* call not valid in the object's current state */
+    OE_ERR_INTERNAL = 5
+};
+
+/* Element ids = reference enum Element (include/Mesh/Elements.h:4-15). */
+enum { OE_TETRA = 3, OE_HEXA = 6, OE_TETRA10 = 7, OE_TETRA20 = 8, OE_HEXA27 = 9 };
+/* Material ids = reference enum Material (include/Dynamic/FEM/ContinuousMaterial.h:8-10). */
+enum { OE_NEOHOOKE = 2, OE_STABLE_NEOHOOKE = 3 };
+/* Mass_Distribution (include/Dynamic/FEM/FEM_Shape.h:693-696). */
+enum { OE_MASS_UNIFORM = 0, OE_MASS_FEMSHAPE = 1 };
+/* Integrator family. */
+enum { OE_SOLVER_VBD = 0, OE_SOLVER_XPBD = 1, OE_SOLVER_EXPLICIT = 2 };
+/* VBD_Version (include/GPU/VBD/GPU_VBD_FEM.h:14-16). */
+enum { OE_VBD_BASE = 0, OE_VBD_THREADED_QUADRATURE = 1, OE_VBD_REDUCTION_SYMMETRY = 2, OE_VBD_BETTER_COLORATION = 3 };
+/* XPBD_FEM_VERSION (include/GPU/PBD/GPU_PBD_FEM.h:11-13). */
+enum { OE_XPBD_XPBD = 0, OE_XPBD_QUADRATURE = 1, OE_XPBD_OPTISHARED = 2, OE_XPBD_OPTIGROUP = 3 };
+
+typedef struct oe_engine oe_engine;
+typedef struct oe_scene oe_scene;
+
+/* Scene description: the arguments of Cuda_{VBD,XPBD}_FEM_Dynamic /
+ * Cuda_FEM_Explicit_Dynamic plus the mesh they are attached to. */
+typedef struct oe_scene_desc {
+    int32_t element;            /* OE_TETRA ... OE_HEXA27 */
+    int32_t n_vertices;
+    const float* xyz;           /* 3 * n_vertices */
+    int32_t n_elements;
+    const int32_t* topology;    /* S * n_elements */
+    int32_t material;           /* OE_NEOHOOKE / OE_STABLE_NEOHOOKE */
+    float young, poisson, density;
+    int32_t mass_distribution;  /* OE_MASS_* */
+    int32_t solver;             /* OE_SOLVER_* */
+    int32_t iterations;         /* VBD iterations per substep */
+    float rho;                  /* VBD Chebyshev spectral radius, 0 = off */
+    float damping;              /* VBD / explicit damping */
+    float global_damping;       /* XPBD velocity damping */
+    int32_t vbd_version;        /* OE_VBD_* */
+    int32_t target_color;       /* OE_VBD_BETTER_COLORATION target, > 0 */
+    int32_t xpbd_version;       /* OE_XPBD_* */
+    float gravity[3];
+    uint64_t shuffle_seed;      /* 0: ascending colour order; else seeded shuffle per sweep */
+} oe_scene_desc;
+
+/* Fill a description with the reference defaults (SNH, VBD v3 semantics, g = -9.81 y). */
+void oe_scene_desc_default(oe_scene_desc* d);
+
+const char* oe_last_error(void);
+const char* oe_version(void);
+
+/* ---------------------------------------------------------------- engine */
+/* budget_bytes = 0: no budget. */
+oe_status oe_engine_create(int32_t device, uint64_t budget_bytes, oe_engine** out);
+oe_status oe_engine_destroy(oe_engine* e);
+oe_status oe_engine_synchronize(oe_engine* e);
+oe_status oe_engine_memory(oe_engine* e, uint64_t* used_bytes, uint64_t* budget_bytes);
+
+/* ------------------------------------------------------------------ scene */
+oe_status oe_scene_create(oe_engine* e, const oe_scene_desc* desc, oe_scene** out);
+oe_status oe_scene_destroy(oe_scene* s);
+/* GPU_Fix_Constraint with an explicit vertex list. */
+oe_status oe_scene_fix_vertices(oe_scene* s, const int32_t* ids, int32_t n);
+/* GPU_Fix_Constraint(geometry, Plane(o, n)): vertices with dot(x - o, n) > 0. */
+oe_status oe_scene_fix_plane(oe_scene* s, const float o[3], const float n[3], int32_t* n_fixed);
+/* GPU_Box_Limit(pmin, pmax). */
+oe_status oe_scene_box_limit(oe_scene* s, const float pmin[3], const float pmax[3]);
+/* Cuda_ParticleSystem_Dynamics::update: substeps x integrator.step(dt / substeps). Asynchronous. */
+oe_status oe_scene_step(oe_scene* s, float dt, int32_t substeps);
+/* Capture the substep loop into a HIP graph and replay it (1) or launch eagerly (0). */
+oe_status oe_scene_set_graph(oe_scene* s, int32_t enable);
+/* Gather tile: lanes per vertex (4..64, power of two), quadrature points per lane. */
+oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl);
+oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl);
+
+/* Counts: vertices, elements, nodes per element, quadrature points, colours. */
+oe_status oe_scene_info(oe_scene* s, int32_t* n_vertices, int32_t* n_elements, int32_t* nodes_per_element,
+                        int32_t* quadrature, int32_t* n_colors, uint64_t* device_bytes);
+/* Read-back (synchronising).  Arrays of 3*N floats / N values. */
+oe_status oe_scene_get_positions(oe_scene* s, float* xyz);
+oe_status oe_scene_get_last_positions(oe_scene* s, float* xyz);
+oe_status oe_scene_get_init_positions(oe_scene* s, float* xyz);
+oe_status oe_scene_get_velocities(oe_scene* s, float* xyz);
+oe_status oe_scene_get_forces(oe_scene* s, float* xyz);
+oe_status oe_scene_get_masses(oe_scene* s, float* m);
+oe_status oe_scene_get_masks(oe_scene* s, int32_t* mask);
+/* Colour of every vertex (VBD) or element (XPBD). */
+oe_status oe_scene_get_colors(oe_scene* s, int32_t* colors);
+/* State export / import (p, last_p, v; any pointer may be NULL on import). */
+oe_status oe_scene_set_state(oe_scene* s, const float* p, const float* last_p, const float* v);
+/* Gather-kernel timing with HIP events around every colour launch. */
+oe_status oe_scene_set_timing(oe_scene* s, int32_t enable);
+oe_status oe_scene_get_timing(oe_scene* s, double* total_ms, int64_t* launches);
+
+/* ------------------------------------------------- host-side (no GPU needed) */
+/* Beam of nx*ny*nz cells of size sx*sy*sz; pass NULL arrays to query sizes. */
+oe_status oe_host_beam_mesh(int32_t element, int32_t nx, int32_t ny, int32_t nz, float sx, float sy, float sz,
+                            int32_t* n_vertices, int32_t* n_elements, float* xyz, int32_t* topology);
+/* Nodes per element and quadrature points; dN at the quadrature points (Q*S*3). */
+oe_status oe_host_shape(int32_t element, int32_t* S, int32_t* Q, double* dN, double* weights, double* qpoints);
+/* DSAT colouring of the primal graph (VBD); target_color > 0 applies ColorReduction. */
+oe_status oe_host_color_primal(int32_t element, const int32_t* topology, int32_t n_elements, int32_t n_vertices,
+                               int32_t target_color, int32_t* colors, int32_t* n_colors, int64_t* conflicts);
+/* Greedy smallest-last colouring of the element (vertex-dual) graph (XPBD). */
+oe_status oe_host_color_dual(int32_t element, const int32_t* topology, int32_t n_elements, int32_t n_vertices,
+                             int32_t* colors, int32_t* n_colors, int64_t* conflicts);
+/* JX_inv (column-major) + V per (element, quadrature point): 10 floats each. */
+oe_status oe_host_fem_constants(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
+                                int32_t n_elements, float* jxv);
+oe_status oe_host_masses(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
+                         int32_t n_elements, float density, int32_t distribution, float* masses);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTOPUS_ENGINE_H */

@@ -1,0 +1,301 @@
+"""MI355X soft-body FEM stepping engine (gfx950, HIP) - Python front end.
+
+The compute path is liboe.so (C ABI in include/octopus_engine.h); this module
+only converts numpy arrays to pointers.  The object names follow the
+reference's script components so that a scene reads like the reference's:
+
+    Cuda_VBD_FEM_Dynamic(density, distrib, young, poisson, material,
+                         iteration, sub_iteration, damping, rho, version, target_color)
+        -> vbd_scene(...)          (include/Script/Dynamic/Cuda_VBD_FEM_Dynamic.h:10-15)
+    Cuda_XPBD_FEM_Dynamic(density, distrib, young, poisson, material,
+                          sub_it, damping, coupled, version)
+        -> xpbd_scene(...)         (include/Script/Dynamic/Cuda_XPBD_FEM_Dynamic.h:13-19)
+    Cuda_FEM_Explicit_Dynamic(sub_it, density, distrib, young, poisson,
+                              material, damping, integrator)
+        -> explicit_scene(...)     (include/Script/Dynamic/Cuda_FEM_Explicit_Dynamic.h:5-21)
+    Cuda_Constraint_Rigid_Controller(Plane(o, n))  -> Scene.fix_plane(o, n)
+    update()                        -> Scene.update() = step(dt, sub_iterations)
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import (HEXA, HEXA27, MASS_FEMSHAPE, MASS_UNIFORM, NEOHOOKE, NODES, SOLVER_EXPLICIT, SOLVER_VBD,
+                   SOLVER_XPBD, STABLE_NEOHOOKE, TETRA, TETRA10, TETRA20, VBD_BASE, VBD_BETTER_COLORATION,
+                   VBD_REDUCTION_SYMMETRY, VBD_THREADED_QUADRATURE, XPBD_OPTIGROUP, XPBD_OPTISHARED,
+                   XPBD_QUADRATURE, XPBD_XPBD, OEError)
+
+__all__ = [
+    "Engine", "Scene", "beam_mesh", "vbd_scene", "xpbd_scene", "explicit_scene", "OEError",
+    "host_shape", "host_color_primal", "host_color_dual", "host_fem_constants", "host_masses",
+    "TETRA", "HEXA", "TETRA10", "TETRA20", "HEXA27", "NEOHOOKE", "STABLE_NEOHOOKE",
+    "MASS_UNIFORM", "MASS_FEMSHAPE", "SOLVER_VBD", "SOLVER_XPBD", "SOLVER_EXPLICIT",
+    "VBD_BASE", "VBD_THREADED_QUADRATURE", "VBD_REDUCTION_SYMMETRY", "VBD_BETTER_COLORATION",
+    "XPBD_XPBD", "XPBD_QUADRATURE", "XPBD_OPTISHARED", "XPBD_OPTIGROUP", "NODES",
+]
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ------------------------------------------------------------------ host side
+def beam_mesh(element: int, cells=(16, 4, 4), size=(1.0, 0.25, 0.25)):
+    """Structured beam (reference BeamMeshGenerator + converters): (xyz (N,3), topology (E,S))."""
+    nv, ne = C.c_int32(), C.c_int32()
+    L.check(L.lib.oe_host_beam_mesh(element, *cells, *[float(s) for s in size], C.byref(nv), C.byref(ne), None, None))
+    xyz = np.empty((nv.value, 3), np.float32)
+    topo = np.empty((ne.value, NODES[element]), np.int32)
+    L.check(L.lib.oe_host_beam_mesh(element, *cells, *[float(s) for s in size], None, None, L.fptr(xyz), L.iptr(topo)))
+    return xyz, topo
+
+
+def host_shape(element: int):
+    S, Q = C.c_int32(), C.c_int32()
+    L.check(L.lib.oe_host_shape(element, C.byref(S), C.byref(Q), None, None, None))
+    dN = np.empty((Q.value, S.value, 3), np.float64)
+    w = np.empty(Q.value, np.float64)
+    qp = np.empty((Q.value, 3), np.float64)
+    L.check(L.lib.oe_host_shape(element, None, None, L.dptr(dN), L.dptr(w), L.dptr(qp)))
+    return dN, w, qp
+
+
+def host_color_primal(element: int, topo, n_vertices: int, target_color: int = -1):
+    topo = _i32(topo)
+    colors = np.empty(n_vertices, np.int32)
+    nc, nconf = C.c_int32(), C.c_int64()
+    L.check(L.lib.oe_host_color_primal(element, L.iptr(topo), topo.shape[0], n_vertices, target_color,
+                                       L.iptr(colors), C.byref(nc), C.byref(nconf)))
+    return colors, nc.value, nconf.value
+
+
+def host_color_dual(element: int, topo, n_vertices: int):
+    topo = _i32(topo)
+    colors = np.empty(topo.shape[0], np.int32)
+    nc, nconf = C.c_int32(), C.c_int64()
+    L.check(L.lib.oe_host_color_dual(element, L.iptr(topo), topo.shape[0], n_vertices, L.iptr(colors),
+                                     C.byref(nc), C.byref(nconf)))
+    return colors, nc.value, nconf.value
+
+
+def host_fem_constants(element: int, xyz, topo):
+    xyz, topo = _f32(xyz), _i32(topo)
+    Q = host_shape(element)[1].shape[0]
+    out = np.empty((topo.shape[0], Q, 10), np.float32)
+    L.check(L.lib.oe_host_fem_constants(element, L.fptr(xyz), xyz.shape[0], L.iptr(topo), topo.shape[0], L.fptr(out)))
+    return out
+
+
+def host_masses(element: int, xyz, topo, density: float, distribution: int):
+    xyz, topo = _f32(xyz), _i32(topo)
+    out = np.empty(xyz.shape[0], np.float32)
+    L.check(L.lib.oe_host_masses(element, L.fptr(xyz), xyz.shape[0], L.iptr(topo), topo.shape[0], density,
+                                 distribution, L.fptr(out)))
+    return out
+
+
+# ------------------------------------------------------------------ device side
+class Engine:
+    """One GPU, one stream, one device-memory budget (bytes, 0 = none)."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib.oe_engine_create(device, budget_bytes, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        L.check(L.lib.oe_engine_synchronize(self._h))
+
+    def memory(self):
+        used, budget = C.c_uint64(), C.c_uint64()
+        L.check(L.lib.oe_engine_memory(self._h, C.byref(used), C.byref(budget)))
+        return used.value, budget.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib.oe_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Mesh + particle system + integrator (+ FEM dynamic + constraints)."""
+
+    def __init__(self, engine: Engine, element: int, xyz, topo, *, material=STABLE_NEOHOOKE, young=1e6,
+                 poisson=0.45, density=1000.0, distribution=MASS_FEMSHAPE, solver=SOLVER_VBD, iterations=10,
+                 rho=0.0, damping=0.0, global_damping=0.0, vbd_version=VBD_REDUCTION_SYMMETRY, target_color=-1,
+                 xpbd_version=XPBD_OPTISHARED, gravity=(0.0, -9.81, 0.0), shuffle_seed=0, sub_iterations=1):
+        self.engine = engine
+        self._xyz = _f32(xyz).reshape(-1, 3)
+        self._topo = _i32(topo).reshape(-1, NODES[element])
+        d = L.SceneDesc()
+        L.lib.oe_scene_desc_default(C.byref(d))
+        d.element = element
+        d.n_vertices = self._xyz.shape[0]
+        d.xyz = L.fptr(self._xyz)
+        d.n_elements = self._topo.shape[0]
+        d.topology = L.iptr(self._topo)
+        d.material = material
+        d.young, d.poisson, d.density = young, poisson, density
+        d.mass_distribution = distribution
+        d.solver = solver
+        d.iterations = iterations
+        d.rho, d.damping, d.global_damping = rho, damping, global_damping
+        d.vbd_version, d.target_color, d.xpbd_version = vbd_version, target_color, xpbd_version
+        d.gravity[:] = list(gravity)
+        d.shuffle_seed = shuffle_seed
+        h = C.c_void_p()
+        L.check(L.lib.oe_scene_create(engine.handle, C.byref(d), C.byref(h)))
+        self._h = h
+        self.element = element
+        self.sub_iterations = sub_iterations
+        nv, ne, S, Q, nc, nb = (C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint64())
+        L.check(L.lib.oe_scene_info(h, C.byref(nv), C.byref(ne), C.byref(S), C.byref(Q), C.byref(nc), C.byref(nb)))
+        self.n_vertices, self.n_elements = nv.value, ne.value
+        self.S, self.Q, self.n_colors, self.device_bytes = S.value, Q.value, nc.value, nb.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    # constraints
+    def fix_plane(self, origin, normal) -> int:
+        n = C.c_int32()
+        L.check(L.lib.oe_scene_fix_plane(self._h, L.fptr(_f32(origin)), L.fptr(_f32(normal)), C.byref(n)))
+        return n.value
+
+    def fix_vertices(self, ids):
+        ids = _i32(ids)
+        L.check(L.lib.oe_scene_fix_vertices(self._h, L.iptr(ids), ids.size))
+
+    def box_limit(self, pmin=(-10.0, -5.0, -10.0), pmax=(10.0, 5.0, 10.0)):
+        L.check(L.lib.oe_scene_box_limit(self._h, L.fptr(_f32(pmin)), L.fptr(_f32(pmax))))
+
+    # stepping
+    def step(self, dt: float, substeps: int = 1):
+        L.check(L.lib.oe_scene_step(self._h, dt, substeps))
+
+    def update(self, dt: float):
+        self.step(dt, self.sub_iterations)
+
+    def set_graph(self, enable: bool):
+        L.check(L.lib.oe_scene_set_graph(self._h, int(enable)))
+
+    def set_tile(self, lanes: int, qpl: int):
+        L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl))
+
+    def get_tile(self):
+        a, b = C.c_int32(), C.c_int32()
+        L.check(L.lib.oe_scene_get_tile(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_timing(self, enable: bool):
+        L.check(L.lib.oe_scene_set_timing(self._h, int(enable)))
+
+    def get_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        L.check(L.lib.oe_scene_get_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # read-back
+    def _get3(self, fn):
+        out = np.empty((self.n_vertices, 3), np.float32)
+        L.check(fn(self._h, L.fptr(out)))
+        return out
+
+    def positions(self):
+        return self._get3(L.lib.oe_scene_get_positions)
+
+    def last_positions(self):
+        return self._get3(L.lib.oe_scene_get_last_positions)
+
+    def init_positions(self):
+        return self._get3(L.lib.oe_scene_get_init_positions)
+
+    def velocities(self):
+        return self._get3(L.lib.oe_scene_get_velocities)
+
+    def forces(self):
+        return self._get3(L.lib.oe_scene_get_forces)
+
+    def masses(self):
+        out = np.empty(self.n_vertices, np.float32)
+        L.check(L.lib.oe_scene_get_masses(self._h, L.fptr(out)))
+        return out
+
+    def masks(self):
+        out = np.empty(self.n_vertices, np.int32)
+        L.check(L.lib.oe_scene_get_masks(self._h, L.iptr(out)))
+        return out
+
+    def colors(self):
+        n = self.n_elements if self._is_xpbd else self.n_vertices
+        out = np.empty(n, np.int32)
+        L.check(L.lib.oe_scene_get_colors(self._h, L.iptr(out)))
+        return out
+
+    @property
+    def _is_xpbd(self):
+        return getattr(self, "_solver", None) == SOLVER_XPBD
+
+    def set_state(self, p=None, last_p=None, v=None):
+        args = [L.fptr(_f32(a)) if a is not None else None for a in (p, last_p, v)]
+        L.check(L.lib.oe_scene_set_state(self._h, *args))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib.oe_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def vbd_scene(engine, element, xyz, topo, density, distrib, young, poisson, material, iteration=30,
+              sub_iteration=1, damping=0.0, rho=0.0, version=VBD_REDUCTION_SYMMETRY, target_color=-1, **kw):
+    """Cuda_VBD_FEM_Dynamic equivalent.  The reference's default version
+    (Better_Coloration with target_color=-1) is rejected there too; the default
+    here is Reduction_Symmetry."""
+    s = Scene(engine, element, xyz, topo, material=material, young=young, poisson=poisson, density=density,
+              distribution=distrib, solver=SOLVER_VBD, iterations=iteration, rho=rho, damping=damping,
+              vbd_version=version, target_color=target_color, sub_iterations=sub_iteration, **kw)
+    s._solver = SOLVER_VBD
+    return s
+
+
+def xpbd_scene(engine, element, xyz, topo, density, distrib, young, poisson, material, sub_it=30, damping=0.0,
+               version=XPBD_OPTISHARED, **kw):
+    """Cuda_XPBD_FEM_Dynamic equivalent (GPU_PBD(global damping) + GPU_PBD_FEM)."""
+    s = Scene(engine, element, xyz, topo, material=material, young=young, poisson=poisson, density=density,
+              distribution=distrib, solver=SOLVER_XPBD, global_damping=damping, xpbd_version=version,
+              sub_iterations=sub_it, **kw)
+    s._solver = SOLVER_XPBD
+    return s
+
+
+def explicit_scene(engine, element, xyz, topo, sub_it, density, distrib, young, poisson, material, damping=0.0,
+                   **kw):
+    """Cuda_FEM_Explicit_Dynamic equivalent with the semi-implicit Euler integrator."""
+    s = Scene(engine, element, xyz, topo, material=material, young=young, poisson=poisson, density=density,
+              distribution=distrib, solver=SOLVER_EXPLICIT, damping=damping, sub_iterations=sub_it, **kw)
+    s._solver = SOLVER_EXPLICIT
+    return s

@@ -1,0 +1,407 @@
+// extern "C" boundary.  Every entry point catches exceptions and maps them to
+// an oe_status; the message is kept per thread for oe_last_error().
+#include "../../include/octopus_engine.h"
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "oe_scene.hpp"
+
+struct oe_engine {
+    std::unique_ptr<oe::Engine> eng;
+};
+struct oe_scene {
+    std::unique_ptr<oe::Scene> scene;
+    oe_engine* owner;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+oe_status fail(oe_status code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+oe_status guard(F&& fn) {
+    try {
+        fn();
+        return OE_OK;
+    } catch (const oe::BudgetError& e) {
+        return fail(OE_ERR_BUDGET, e.what());
+    } catch (const oe::HipError& e) {
+        return fail(OE_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(OE_ERR_INVALID, e.what());
+    } catch (const std::out_of_range& e) {
+        return fail(OE_ERR_INVALID, e.what());
+    } catch (const std::length_error& e) {
+        return fail(OE_ERR_INVALID, e.what());
+    } catch (const std::logic_error& e) {
+        return fail(OE_ERR_STATE, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(OE_ERR_INTERNAL, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(OE_ERR_INTERNAL, e.what());
+    } catch (...) {
+        return fail(OE_ERR_INTERNAL, "unknown error");
+    }
+}
+
+#define OE_REQUIRE(cond, msg) \
+    do {                      \
+        if (!(cond)) throw std::invalid_argument(msg); \
+    } while (0)
+
+oe::HostMesh mesh_from(int32_t element, const float* xyz, int32_t nv, const int32_t* topo, int32_t ne) {
+    OE_REQUIRE(oe::element_supported(element), "unsupported element type");
+    OE_REQUIRE(nv > 0 && ne > 0 && xyz && topo, "empty mesh");
+    oe::HostMesh m;
+    m.element = static_cast<oe::Element>(element);
+    const int S = oe::elem_nb_vertices(m.element);
+    m.xyz.assign(xyz, xyz + 3 * static_cast<size_t>(nv));
+    m.topo.assign(topo, topo + static_cast<size_t>(S) * ne);
+    for (int32_t v : m.topo) OE_REQUIRE(v >= 0 && v < nv, "topology index out of range");
+    return m;
+}
+
+void copy3(std::vector<float>& src, float* dst) { std::memcpy(dst, src.data(), src.size() * sizeof(float)); }
+
+}  // namespace
+
+extern "C" {
+
+void oe_scene_desc_default(oe_scene_desc* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->element = OE_TETRA10;
+    d->material = OE_STABLE_NEOHOOKE;
+    d->young = 1e6f;
+    d->poisson = 0.45f;
+    d->density = 1000.f;
+    d->mass_distribution = OE_MASS_FEMSHAPE;
+    d->solver = OE_SOLVER_VBD;
+    d->iterations = 10;
+    d->vbd_version = OE_VBD_REDUCTION_SYMMETRY;
+    d->target_color = -1;
+    d->xpbd_version = OE_XPBD_OPTISHARED;
+    d->gravity[0] = 0.f;
+    d->gravity[1] = -9.81f;
+    d->gravity[2] = 0.f;
+}
+
+const char* oe_last_error(void) { return g_last_error.c_str(); }
+const char* oe_version(void) { return "octopusengine_amd 0.1 (gfx950)"; }
+
+oe_status oe_engine_create(int32_t device, uint64_t budget_bytes, oe_engine** out) {
+    return guard([&] {
+        OE_REQUIRE(out, "null output");
+        auto* e = new oe_engine();
+        try {
+            e->eng = std::make_unique<oe::Engine>(device, budget_bytes ? budget_bytes : SIZE_MAX);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+oe_status oe_engine_destroy(oe_engine* e) {
+    return guard([&] { delete e; });
+}
+
+oe_status oe_engine_synchronize(oe_engine* e) {
+    return guard([&] {
+        OE_REQUIRE(e, "null engine");
+        OE_HIP(hipStreamSynchronize(e->eng->stream));
+        OE_HIP(hipDeviceSynchronize());
+    });
+}
+
+oe_status oe_engine_memory(oe_engine* e, uint64_t* used, uint64_t* budget) {
+    return guard([&] {
+        OE_REQUIRE(e, "null engine");
+        if (used) *used = e->eng->pool.used.load();
+        if (budget) *budget = e->eng->pool.budget;
+    });
+}
+
+oe_status oe_scene_create(oe_engine* e, const oe_scene_desc* d, oe_scene** out) {
+    return guard([&] {
+        OE_REQUIRE(e && d && out, "null argument");
+        oe::HostMesh m = mesh_from(d->element, d->xyz, d->n_vertices, d->topology, d->n_elements);
+        oe::SceneConfig c;
+        OE_REQUIRE(d->material == OE_NEOHOOKE || d->material == OE_STABLE_NEOHOOKE, "unsupported material");
+        OE_REQUIRE(d->mass_distribution == OE_MASS_UNIFORM || d->mass_distribution == OE_MASS_FEMSHAPE,
+                   "unknown mass distribution");
+        OE_REQUIRE(d->solver >= OE_SOLVER_VBD && d->solver <= OE_SOLVER_EXPLICIT, "unknown solver");
+        OE_REQUIRE(d->vbd_version >= OE_VBD_BASE && d->vbd_version <= OE_VBD_BETTER_COLORATION, "unknown VBD version");
+        OE_REQUIRE(d->xpbd_version >= OE_XPBD_XPBD && d->xpbd_version <= OE_XPBD_OPTIGROUP, "unknown XPBD version");
+        OE_REQUIRE(d->iterations >= 0, "iterations must be >= 0");
+        c.solver = d->solver;
+        c.material = static_cast<oe::Material>(d->material);
+        c.young = d->young;
+        c.poisson = d->poisson;
+        c.density = d->density;
+        c.distribution = static_cast<oe::Mass_Distribution>(d->mass_distribution);
+        c.iterations = d->iterations;
+        c.rho = d->rho;
+        c.damping = d->damping;
+        c.global_damping = d->global_damping;
+        c.vbd_version = d->vbd_version;
+        c.target_color = d->target_color;
+        c.xpbd_version = d->xpbd_version;
+        for (int k = 0; k < 3; ++k) c.gravity[k] = d->gravity[k];
+        c.shuffle_seed = d->shuffle_seed;
+        auto* s = new oe_scene();
+        try {
+            s->scene = std::make_unique<oe::Scene>(e->eng.get(), std::move(m), c);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        s->owner = e;
+        *out = s;
+    });
+}
+
+oe_status oe_scene_destroy(oe_scene* s) {
+    return guard([&] { delete s; });
+}
+
+oe_status oe_scene_fix_vertices(oe_scene* s, const int32_t* ids, int32_t n) {
+    return guard([&] {
+        OE_REQUIRE(s && (ids || n == 0) && n >= 0, "bad argument");
+        s->scene->fix_vertices(std::vector<int>(ids, ids + n));
+    });
+}
+
+oe_status oe_scene_fix_plane(oe_scene* s, const float o[3], const float n[3], int32_t* n_fixed) {
+    return guard([&] {
+        OE_REQUIRE(s && o && n, "bad argument");
+        const std::vector<int> ids = s->scene->select_plane(o, n);
+        s->scene->fix_vertices(ids);
+        if (n_fixed) *n_fixed = static_cast<int32_t>(ids.size());
+    });
+}
+
+oe_status oe_scene_box_limit(oe_scene* s, const float pmin[3], const float pmax[3]) {
+    return guard([&] {
+        OE_REQUIRE(s && pmin && pmax, "bad argument");
+        s->scene->box_limit(pmin, pmax);
+    });
+}
+
+oe_status oe_scene_step(oe_scene* s, float dt, int32_t substeps) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        s->scene->step(dt, substeps);
+    });
+}
+
+oe_status oe_scene_set_graph(oe_scene* s, int32_t enable) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        s->scene->set_graph(enable != 0);
+    });
+}
+
+oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        oe::Scene& sc = *s->scene;
+        OE_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "lanes must be 4..64, power of 2");
+        const int Q = sc.fem->Q;
+        OE_REQUIRE(qpl >= 1 && Q % qpl == 0, "quadrature points per lane must divide Q");
+        OE_REQUIRE(qpl == 1 || qpl == Q || (Q == 4 && qpl == 2) || (Q == 27 && (qpl == 3 || qpl == 9)),
+                   "tile not instantiated");
+        sc.set_graph(false);
+        if (sc.vbd_fem) sc.vbd_fem->tile = oe::GatherTile{lanes, qpl};
+        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl};
+    });
+}
+
+oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        oe::GatherTile t{0, 0};
+        if (s->scene->vbd_fem) t = s->scene->vbd_fem->tile;
+        if (s->scene->explicit_fem) t = s->scene->explicit_fem->tile;
+        if (lanes) *lanes = t.L;
+        if (qpl) *qpl = t.QL;
+    });
+}
+
+oe_status oe_scene_info(oe_scene* s, int32_t* nv, int32_t* ne, int32_t* S, int32_t* Q, int32_t* nc, uint64_t* bytes) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        oe::Scene& sc = *s->scene;
+        if (nv) *nv = sc.mesh.nb_vertices();
+        if (ne) *ne = sc.mesh.nb_elements();
+        if (S) *S = sc.fem->S;
+        if (Q) *Q = sc.fem->Q;
+        if (nc) {
+            *nc = 0;
+            if (sc.vbd_fem) *nc = sc.vbd_fem->colors.nb_color;
+            if (sc.pbd_fem) *nc = sc.pbd_fem->colors.nb_color;
+        }
+        if (bytes) *bytes = sc.device_bytes();
+    });
+}
+
+#define OE_GETTER3(name, method)                             \
+    oe_status name(oe_scene* s, float* xyz) {                \
+        return guard([&] {                                   \
+            OE_REQUIRE(s && xyz, "bad argument");            \
+            std::vector<float> h;                            \
+            s->scene->ps->method(h);                         \
+            copy3(h, xyz);                                   \
+        });                                                  \
+    }
+
+OE_GETTER3(oe_scene_get_positions, get_position)
+OE_GETTER3(oe_scene_get_last_positions, get_prev_position)
+OE_GETTER3(oe_scene_get_init_positions, get_init_position)
+OE_GETTER3(oe_scene_get_velocities, get_velocity)
+OE_GETTER3(oe_scene_get_forces, get_forces)
+
+oe_status oe_scene_get_masses(oe_scene* s, float* m) {
+    return guard([&] {
+        OE_REQUIRE(s && m, "bad argument");
+        std::vector<float> h;
+        s->scene->ps->get_mass(h);
+        std::memcpy(m, h.data(), h.size() * sizeof(float));
+    });
+}
+
+oe_status oe_scene_get_masks(oe_scene* s, int32_t* mask) {
+    return guard([&] {
+        OE_REQUIRE(s && mask, "bad argument");
+        std::vector<int> h;
+        s->scene->ps->get_masks(h);
+        std::memcpy(mask, h.data(), h.size() * sizeof(int32_t));
+    });
+}
+
+oe_status oe_scene_get_colors(oe_scene* s, int32_t* colors) {
+    return guard([&] {
+        OE_REQUIRE(s && colors, "bad argument");
+        const oe::Scene& sc = *s->scene;
+        const std::vector<int32_t>* c = nullptr;
+        if (sc.vbd_fem) c = &sc.vbd_fem->colors.colors;
+        if (sc.pbd_fem) c = &sc.pbd_fem->colors.colors;
+        if (!c) throw std::logic_error("scene has no colouring (explicit solver)");
+        std::memcpy(colors, c->data(), c->size() * sizeof(int32_t));
+    });
+}
+
+oe_status oe_scene_set_state(oe_scene* s, const float* p, const float* last_p, const float* v) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        s->scene->ps->set_state(p, last_p, v);
+    });
+}
+
+oe_status oe_scene_set_timing(oe_scene* s, int32_t enable) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        if (!s->scene->vbd_fem) throw std::logic_error("gather timing is available on VBD scenes");
+        s->scene->vbd_fem->time_gather = enable != 0;
+    });
+}
+
+oe_status oe_scene_get_timing(oe_scene* s, double* total_ms, int64_t* launches) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        if (!s->scene->vbd_fem) throw std::logic_error("gather timing is available on VBD scenes");
+        oe::GPU_VBD_FEM* f = s->scene->vbd_fem;
+        f->collect_gather_timing();
+        if (total_ms) *total_ms = f->gather_ms;
+        if (launches) *launches = f->gather_launches;
+        f->gather_ms = 0.0;
+        f->gather_launches = 0;
+    });
+}
+
+// ------------------------------------------------------------------ host side
+oe_status oe_host_beam_mesh(int32_t element, int32_t nx, int32_t ny, int32_t nz, float sx, float sy, float sz,
+                            int32_t* n_vertices, int32_t* n_elements, float* xyz, int32_t* topology) {
+    return guard([&] {
+        OE_REQUIRE(oe::element_supported(element), "unsupported element type");
+        const oe::HostMesh m = oe::beam_mesh(static_cast<oe::Element>(element), nx, ny, nz, sx, sy, sz);
+        if (n_vertices) *n_vertices = m.nb_vertices();
+        if (n_elements) *n_elements = m.nb_elements();
+        if (xyz) std::memcpy(xyz, m.xyz.data(), m.xyz.size() * sizeof(float));
+        if (topology) std::memcpy(topology, m.topo.data(), m.topo.size() * sizeof(int32_t));
+    });
+}
+
+oe_status oe_host_shape(int32_t element, int32_t* S, int32_t* Q, double* dN, double* weights, double* qpoints) {
+    return guard([&] {
+        OE_REQUIRE(oe::element_supported(element), "unsupported element type");
+        const oe::ShapeTable& t = oe::shape_table(static_cast<oe::Element>(element));
+        if (S) *S = t.S;
+        if (Q) *Q = t.Q;
+        if (dN) std::memcpy(dN, t.dN.data(), t.dN.size() * sizeof(double));
+        if (weights) std::memcpy(weights, t.weights.data(), t.weights.size() * sizeof(double));
+        if (qpoints) std::memcpy(qpoints, t.qpoints.data(), t.qpoints.size() * sizeof(double));
+    });
+}
+
+oe_status oe_host_color_primal(int32_t element, const int32_t* topology, int32_t ne, int32_t nv, int32_t target,
+                               int32_t* colors, int32_t* n_colors, int64_t* conflicts) {
+    return guard([&] {
+        OE_REQUIRE(oe::element_supported(element) && topology && ne > 0, "bad argument");
+        const oe::Element el = static_cast<oe::Element>(element);
+        const int S = oe::elem_nb_vertices(el);
+        std::vector<int32_t> topo(topology, topology + static_cast<size_t>(S) * ne);
+        const oe::MultiGraph g = oe::build_primal_graph(el, topo, nv);
+        oe::Coloration c = oe::color_dsat(g);
+        if (target > 0) oe::color_reduction(g, c, target);
+        if (colors) std::memcpy(colors, c.colors.data(), c.colors.size() * sizeof(int32_t));
+        if (n_colors) *n_colors = c.nb_color;
+        if (conflicts) *conflicts = oe::count_conflicts(g, c);
+    });
+}
+
+oe_status oe_host_color_dual(int32_t element, const int32_t* topology, int32_t ne, int32_t nv, int32_t* colors,
+                             int32_t* n_colors, int64_t* conflicts) {
+    return guard([&] {
+        OE_REQUIRE(oe::element_supported(element) && topology && ne > 0, "bad argument");
+        const oe::Element el = static_cast<oe::Element>(element);
+        const int S = oe::elem_nb_vertices(el);
+        std::vector<int32_t> topo(topology, topology + static_cast<size_t>(S) * ne);
+        const oe::MultiGraph g = oe::build_dual_vertex_graph(el, topo, nv);
+        const oe::Coloration c = oe::color_greedy_slf(g);
+        if (colors) std::memcpy(colors, c.colors.data(), c.colors.size() * sizeof(int32_t));
+        if (n_colors) *n_colors = c.nb_color;
+        if (conflicts) *conflicts = oe::count_conflicts(g, c);
+    });
+}
+
+oe_status oe_host_fem_constants(int32_t element, const float* xyz, int32_t nv, const int32_t* topology, int32_t ne,
+                                float* jxv) {
+    return guard([&] {
+        const oe::HostMesh m = mesh_from(element, xyz, nv, topology, ne);
+        const oe::FemConstants fc = oe::build_fem_constants(m);
+        if (jxv) std::memcpy(jxv, fc.jxv.data(), fc.jxv.size() * sizeof(float));
+    });
+}
+
+oe_status oe_host_masses(int32_t element, const float* xyz, int32_t nv, const int32_t* topology, int32_t ne,
+                         float density, int32_t distribution, float* masses) {
+    return guard([&] {
+        const oe::HostMesh m = mesh_from(element, xyz, nv, topology, ne);
+        const oe::FemConstants fc = oe::build_fem_constants(m);
+        const std::vector<float> ms =
+            oe::compute_fem_mass(m, fc, density, static_cast<oe::Mass_Distribution>(distribution));
+        if (masses) std::memcpy(masses, ms.data(), ms.size() * sizeof(float));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// HIP plumbing: error checking, an owning device buffer with engine-level
+// memory accounting, and the plain-old-data views passed to kernels by value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace oe {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+struct BudgetError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define OE_HIP(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            throw ::oe::HipError(_e, std::string(#expr " failed: ") + hipGetErrorString(_e) +  \
+                                          " at " __FILE__ ":" + std::to_string(__LINE__));      \
+    } while (0)
+
+#define OE_LAUNCH_CHECK() OE_HIP(hipGetLastError())
+
+// Device memory accounting shared by every buffer of one engine: allocation
+// beyond `budget` bytes is refused before hipMalloc is called.
+struct MemoryPool {
+    std::atomic<size_t> used{0};
+    size_t budget = SIZE_MAX;
+    void reserve(size_t bytes) {
+        size_t cur = used.load();
+        while (true) {
+            if (cur + bytes > budget)
+                throw BudgetError("device memory budget exceeded (" + std::to_string(cur + bytes) + " > " +
+                                  std::to_string(budget) + " bytes)");
+            if (used.compare_exchange_weak(cur, cur + bytes)) return;
+        }
+    }
+    void release(size_t bytes) { used.fetch_sub(bytes); }
+};
+
+template <typename T>
+class DBuf {
+public:
+    DBuf() = default;
+    DBuf(MemoryPool* pool, size_t n) { allocate(pool, n); }
+    DBuf(MemoryPool* pool, const std::vector<T>& h) {
+        allocate(pool, h.size());
+        upload(h);
+    }
+    ~DBuf() { reset(); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept { swap(o); }
+    DBuf& operator=(DBuf&& o) noexcept {
+        reset();
+        swap(o);
+        return *this;
+    }
+
+    void allocate(MemoryPool* pool, size_t n) {
+        reset();
+        const size_t bytes = (n ? n : 1) * sizeof(T);
+        if (pool) pool->reserve(bytes);
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            if (pool) pool->release(bytes);
+            throw HipError(e, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        ptr_ = static_cast<T*>(p);
+        n_ = n;
+        bytes_ = bytes;
+        pool_ = pool;
+    }
+    // Set-up path: synchronous, the host vector may be freed on return.
+    void upload(const std::vector<T>& h, hipStream_t s = nullptr) {
+        if (h.size() > n_) throw std::out_of_range("upload larger than buffer");
+        if (!h.empty()) OE_HIP(hipMemcpyAsync(ptr_, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        OE_HIP(hipStreamSynchronize(s));
+    }
+    void upload_raw(const T* h, size_t n, hipStream_t s) {
+        if (n > n_) throw std::out_of_range("upload larger than buffer");
+        if (n) OE_HIP(hipMemcpyAsync(ptr_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(std::vector<T>& h, hipStream_t s) const {
+        h.resize(n_);
+        if (n_) OE_HIP(hipMemcpyAsync(h.data(), ptr_, n_ * sizeof(T), hipMemcpyDeviceToHost, s));
+        OE_HIP(hipStreamSynchronize(s));
+    }
+    void reset() {
+        if (ptr_) {
+            (void)hipFree(ptr_);
+            if (pool_) pool_->release(bytes_);
+        }
+        ptr_ = nullptr;
+        n_ = 0;
+        bytes_ = 0;
+        pool_ = nullptr;
+    }
+    T* get() const { return ptr_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return bytes_; }
+
+private:
+    void swap(DBuf& o) {
+        std::swap(ptr_, o.ptr_);
+        std::swap(n_, o.n_);
+        std::swap(bytes_, o.bytes_);
+        std::swap(pool_, o.pool_);
+    }
+    T* ptr_ = nullptr;
+    size_t n_ = 0;
+    size_t bytes_ = 0;
+    MemoryPool* pool_ = nullptr;
+};
+
+// Particle state, structure of arrays.  Each field is its own array; a
+// position is one 16-byte record so that a gather of a node's position is one
+// vector load.
+struct PSView {
+    int n;
+    float4* p;
+    float4* last_p;
+    float4* init_p;
+    float4* v;
+    float4* f;
+    const float* m;
+    const float* w;
+    int* mask;
+};
+
+// Per-mesh FEM data (reference GPU_FEM_Pameters, include/GPU/GPU_FEM.h:49-59).
+struct FemView {
+    int E, S, Q;
+    const int* topo;     // E * S
+    const float* jxv;    // E * Q * 10 : JX_inv column-major, then V
+    const float* dN;     // Q * S * 3
+};
+
+// Colour-major owners CSR (reference GPU_Owners_Parameters,
+// include/GPU/VBD/GPU_VBD_FEM.h:36-42).
+struct OwnersView {
+    const int* slot_vid;
+    const int2* slot_nbo;   // (incident element count, offset into own)
+    const int* own;         // (eid << 5) | local node
+};
+
+struct MaterialView {
+    int material;
+    float lambda, mu, alpha;   // alpha = 1 (NeoHooke) or 1 + mu/lambda (Stable NeoHooke)
+};
+
+}  // namespace oe

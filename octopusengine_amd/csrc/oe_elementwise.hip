@@ -1,0 +1,186 @@
+// Per-vertex kernels of the integrators and constraints.  Each is one pass
+// over SoA particle state with coalesced 16-byte accesses; they are launch- or
+// HBM-bound and carry no reuse.
+//   VBD predictor             src/GPU/VBD/GPU_VBD.cu:6-18
+//   VBD velocity              src/GPU/VBD/GPU_VBD.cu:20-25
+//   Chebyshev acceleration    src/GPU/VBD/GPU_VBD.cu:27-35
+//   semi-implicit Euler       src/GPU/GPU_Integrator.cu:27-34
+//   XPBD velocity + damping   src/GPU/PBD/GPU_PBD.cu:8-18
+//   fixed vertices            src/GPU/GPU_Constraint.cu:8-19
+//   box limit                 src/GPU/GPU_Constraint.cu:70-81
+//   Jacobi colour copy        src/GPU/VBD/GPU_VBD_FEM.cu:402-413
+#include "oe_elementwise.hpp"
+
+namespace oe {
+
+namespace {
+
+constexpr int kBlock = 256;
+inline int grid_for(long long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ float4 f4(float x, float y, float z, float w) { return make_float4(x, y, z, w); }
+
+// y = p + ((p - last_p) * (1/dt) + (g + f w) dt) dt ; last_p = p ; p = y ; f = 0
+__global__ __launch_bounds__(kBlock) void k_vbd_predict(PSView ps, float4* y, float4* prev_it_p, float dt, float gx,
+                                                        float gy, float gz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    const float4 p = ps.p[i];
+    const float4 lp = ps.last_p[i];
+    const float4 f = ps.f[i];
+    const float w = ps.w[i];
+    if (prev_it_p) prev_it_p[i] = p;
+    const float inv_dt = 1.f / dt;
+    const float ax = gx + f.x * w, ay = gy + f.y * w, az = gz + f.z * w;
+    const float yx = p.x + ((p.x - lp.x) * inv_dt + ax * dt) * dt;
+    const float yy = p.y + ((p.y - lp.y) * inv_dt + ay * dt) * dt;
+    const float yz = p.z + ((p.z - lp.z) * inv_dt + az * dt) * dt;
+    const float4 yv = f4(yx, yy, yz, p.w);
+    y[i] = yv;
+    ps.last_p[i] = p;
+    ps.p[i] = yv;
+    ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_vbd_velocity(PSView ps, float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    const float4 p = ps.p[i], lp = ps.last_p[i];
+    ps.v[i] = f4((p.x - lp.x) / dt, (p.y - lp.y) / dt, (p.z - lp.z) / dt, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_chebyshev(PSView ps, float4* prev_it_p, float4* prev_it2_p, int it,
+                                                      float omega) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    float4 p = ps.p[i];
+    if (it >= 2) {
+        const float4 q = prev_it2_p[i];
+        p = f4(q.x + omega * (p.x - q.x), q.y + omega * (p.y - q.y), q.z + omega * (p.z - q.z), p.w);
+        ps.p[i] = p;
+    }
+    prev_it2_p[i] = prev_it_p[i];
+    prev_it_p[i] = p;
+}
+
+// last_p = p ; v += (g + f w) dt ; p += v dt ; f = 0
+__global__ __launch_bounds__(kBlock) void k_semi_implicit(PSView ps, float dt, float gx, float gy, float gz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    float4 p = ps.p[i];
+    float4 v = ps.v[i];
+    const float4 f = ps.f[i];
+    const float w = ps.w[i];
+    ps.last_p[i] = p;
+    v.x += (gx + f.x * w) * dt;
+    v.y += (gy + f.y * w) * dt;
+    v.z += (gz + f.z * w) * dt;
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    ps.v[i] = v;
+    ps.p[i] = p;
+    ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbd_velocity(PSView ps, float dt, float global_damping) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    const float4 p = ps.p[i], lp = ps.last_p[i];
+    float vx = (p.x - lp.x) / dt, vy = (p.y - lp.y) / dt, vz = (p.z - lp.z) / dt;
+    const float nv = sqrtf(vx * vx + vy * vy + vz * vz);
+    if (nv > 1e-7f) {
+        const float coef = global_damping * dt * ps.w[i];
+        const float damp = -nv * (coef > 1.f ? 1.f : coef);
+        const float inv = 1.f / nv;
+        vx += vx * inv * damp;
+        vy += vy * inv * damp;
+        vz += vz * inv * damp;
+    }
+    ps.v[i] = f4(vx, vy, vz, 0.f);
+}
+
+// target = offset + com + R (init_p - com);  p = target, v = f = 0, mask = 0
+__global__ __launch_bounds__(kBlock) void k_fix(PSView ps, const int* ids, int n, FixTransform t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int id = ids[i];
+    const float4 x0 = ps.init_p[id];
+    const float dx = x0.x - t.com[0], dy = x0.y - t.com[1], dz = x0.z - t.com[2];
+    // R is column-major; R d = col0 dx + col1 dy + col2 dz
+    const float rx = t.R[0] * dx + t.R[3] * dy + t.R[6] * dz;
+    const float ry = t.R[1] * dx + t.R[4] * dy + t.R[7] * dz;
+    const float rz = t.R[2] * dx + t.R[5] * dy + t.R[8] * dz;
+    const float bx = t.offset[0] + t.com[0], by = t.offset[1] + t.com[1], bz = t.offset[2] + t.com[2];
+    const float4 p = ps.p[id];
+    ps.p[id] = f4(bx + rx, by + ry, bz + rz, p.w);
+    ps.v[id] = f4(0.f, 0.f, 0.f, 0.f);
+    ps.f[id] = f4(0.f, 0.f, 0.f, 0.f);
+    ps.mask[id] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 pmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    if (ps.mask[i] == 0) return;
+    float4 p = ps.p[i];
+    p.x = fminf(fmaxf(p.x, pmin.x), pmax.x);
+    p.y = fminf(fmaxf(p.y, pmin.y), pmax.y);
+    p.z = fminf(fmaxf(p.z, pmin.z), pmax.z);
+    ps.p[i] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* slot_vid, int first, int count,
+                                                        const float4* p_new) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int vid = slot_vid[first + i];
+    if (ps.mask[vid] == 0) return;
+    ps.p[vid] = p_new[vid];
+}
+
+}  // namespace
+
+void launch_vbd_predict(const PSView& ps, float4* y, float4* prev_it_p, float dt, const float g[3], hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_vbd_predict, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, y, prev_it_p, dt, g[0], g[1], g[2]);
+}
+
+void launch_vbd_velocity(const PSView& ps, float dt, hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_vbd_velocity, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, dt);
+}
+
+void launch_chebyshev(const PSView& ps, float4* prev_it_p, float4* prev_it2_p, int it, float omega, hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_chebyshev, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, prev_it_p, prev_it2_p, it, omega);
+}
+
+void launch_semi_implicit(const PSView& ps, float dt, const float g[3], hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_semi_implicit, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, dt, g[0], g[1], g[2]);
+}
+
+void launch_pbd_velocity(const PSView& ps, float dt, float global_damping, hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_pbd_velocity, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, dt, global_damping);
+}
+
+void launch_fix(const PSView& ps, const int* ids, int n, const FixTransform& t, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fix, dim3(grid_for(n)), dim3(kBlock), 0, s, ps, ids, n, t);
+}
+
+void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_box, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, make_float3(pmin[0], pmin[1], pmin[2]),
+                       make_float3(pmax[0], pmax[1], pmax[2]));
+}
+
+void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,
+                        hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_copy_colour, dim3(grid_for(count)), dim3(kBlock), 0, s, ps, slot_vid, first, count, p_new);
+}
+
+}  // namespace oe

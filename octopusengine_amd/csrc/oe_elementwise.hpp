@@ -1,0 +1,27 @@
+// Launchers of the per-vertex integrator and constraint kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "oe_device.hpp"
+
+namespace oe {
+
+// Rigid target of the fixed vertices: target = offset + com + R (x0 - com).
+struct FixTransform {
+    float com[3];
+    float offset[3];
+    float R[9];   // column-major
+};
+
+void launch_vbd_predict(const PSView& ps, float4* y, float4* prev_it_p, float dt, const float g[3], hipStream_t s);
+void launch_vbd_velocity(const PSView& ps, float dt, hipStream_t s);
+void launch_chebyshev(const PSView& ps, float4* prev_it_p, float4* prev_it2_p, int it, float omega, hipStream_t s);
+void launch_semi_implicit(const PSView& ps, float dt, const float g[3], hipStream_t s);
+void launch_pbd_velocity(const PSView& ps, float dt, float global_damping, hipStream_t s);
+void launch_fix(const PSView& ps, const int* ids, int n, const FixTransform& t, hipStream_t s);
+void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipStream_t s);
+void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,
+                        hipStream_t s);
+
+}  // namespace oe

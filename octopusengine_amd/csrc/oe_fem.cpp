@@ -1,0 +1,146 @@
+// Per-(element, quadrature) FEM constants, lumped masses and the colour-major
+// owners CSR.
+//
+// JX_inv and V follow src/GPU/GPU_FEM.cu:176-214 (J = sum_k X_k (x) dN_k,
+// V = |det J| w, JX_inv = J^-1) but are evaluated in double precision and
+// rounded once to float.  Masses follow src/Dynamic/FEM/FEM_Shape.cpp:86-121.
+// The owners CSR follows src/GPU/VBD/GPU_VBD_FEM.cu:594-690: vertices grouped
+// by colour (ascending colour, ascending id inside a colour), each vertex
+// listing its incident elements in ascending element id with the local node
+// index.
+#include "oe_host.hpp"
+
+#include <cmath>
+#include <stdexcept>
+
+namespace oe {
+
+FemConstants build_fem_constants(const HostMesh& m) {
+    const ShapeTable& st = shape_table(m.element);
+    const int S = st.S, Q = st.Q, E = m.nb_elements();
+    FemConstants fc;
+    fc.S = S; fc.Q = Q; fc.E = E;
+    fc.jxv.resize(static_cast<size_t>(E) * Q * 10);
+    fc.dN.resize(static_cast<size_t>(Q) * S * 3);
+    fc.elem_volume.assign(E, 0.0);
+    for (size_t i = 0; i < fc.dN.size(); ++i) fc.dN[i] = static_cast<float>(st.dN[i]);
+    for (int e = 0; e < E; ++e) {
+        const int32_t* t = &m.topo[static_cast<size_t>(e) * S];
+        for (int q = 0; q < Q; ++q) {
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int k = 0; k < S; ++k) {
+                const double* d = &st.dN[(q * S + k) * 3];
+                const float* X = &m.xyz[3 * static_cast<size_t>(t[k])];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) J[i][j] += static_cast<double>(X[i]) * d[j];
+            }
+            const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+            const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+            const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+            if (det == 0.0) throw std::invalid_argument("degenerate element (det J = 0)");
+            // inverse = adj(J) / det, adj(J)[i][j] = cofactor(J)[j][i]
+            double inv[3][3];
+            inv[0][0] = c00 / det;
+            inv[1][0] = c01 / det;
+            inv[2][0] = c02 / det;
+            inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+            inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+            inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+            inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+            inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+            inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+            float* rec = &fc.jxv[(static_cast<size_t>(e) * Q + q) * 10];
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) rec[j * 3 + i] = static_cast<float>(inv[i][j]);  // column-major
+            const double V = std::fabs(det) * st.weights[q];
+            rec[9] = static_cast<float>(V);
+            fc.elem_volume[e] += V;
+        }
+    }
+    return fc;
+}
+
+std::vector<float> compute_fem_mass(const HostMesh& m, const FemConstants& fc, float density,
+                                    Mass_Distribution distrib) {
+    const int N = m.nb_vertices();
+    const int S = fc.S;
+    std::vector<double> mass(N, 0.0);
+    if (distrib == FemShape) {
+        const double vd = static_cast<double>(density) / S;
+        for (int e = 0; e < fc.E; ++e)
+            for (int j = 0; j < S; ++j) mass[m.topo[static_cast<size_t>(e) * S + j]] += vd * fc.elem_volume[e];
+    } else {
+        double vol = 0.0;
+        for (int e = 0; e < fc.E; ++e) vol += fc.elem_volume[e];
+        const double pm = vol * density / N;
+        for (int i = 0; i < N; ++i) mass[i] = pm;
+    }
+    std::vector<float> out(N);
+    for (int i = 0; i < N; ++i) out[i] = static_cast<float>(mass[i]);
+    return out;
+}
+
+void lame_parameters(float young, float poisson, float* lambda, float* mu) {
+    *lambda = young * poisson / ((1.f + poisson) * (1.f - 2.f * poisson));
+    *mu = young / (2.f * (1.f + poisson));
+}
+
+OwnersCSR build_owners(const HostMesh& m, const Coloration& col) {
+    const int N = m.nb_vertices();
+    const int S = elem_nb_vertices(m.element);
+    const int E = m.nb_elements();
+    if (static_cast<int>(col.colors.size()) != N) throw std::invalid_argument("colouring size mismatch");
+    // owner records pack (eid << 5 | local node): eid must fit 26 bits, S <= 32
+    if (E >= (1 << 26) || S > 32) throw std::invalid_argument("mesh too large for the packed owners CSR");
+    // per-vertex incident (eid, ref) lists in ascending eid
+    std::vector<int64_t> cnt(N + 1, 0);
+    for (int32_t v : m.topo) cnt[v + 1]++;
+    for (int i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int32_t> inc(m.topo.size());
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < E; ++e)
+        for (int j = 0; j < S; ++j) {
+            const int32_t v = m.topo[static_cast<size_t>(e) * S + j];
+            inc[fill[v]++] = (e << 5) | j;
+        }
+    OwnersCSR o;
+    const int nc = col.nb_color;
+    o.color_begin.assign(nc + 1, 0);
+    o.color_max_nb.assign(nc, 0);
+    for (int v = 0; v < N; ++v) {
+        const int c = col.colors[v];
+        if (c < 0 || c >= nc) throw std::invalid_argument("colour out of range");
+        o.color_begin[c + 1]++;
+    }
+    for (int c = 0; c < nc; ++c) o.color_begin[c + 1] += o.color_begin[c];
+    std::vector<int32_t> cur(o.color_begin.begin(), o.color_begin.end() - 1);
+    o.slot_vid.resize(N);
+    for (int v = 0; v < N; ++v) o.slot_vid[cur[col.colors[v]]++] = v;
+    o.slot_nb.resize(N);
+    o.slot_off.resize(N);
+    o.own.reserve(m.topo.size());
+    for (int s = 0; s < N; ++s) {
+        const int v = o.slot_vid[s];
+        o.slot_off[s] = static_cast<int32_t>(o.own.size());
+        o.slot_nb[s] = static_cast<int32_t>(cnt[v + 1] - cnt[v]);
+        for (int64_t i = cnt[v]; i < cnt[v + 1]; ++i) o.own.push_back(inc[i]);
+        const int c = col.colors[v];
+        o.color_max_nb[c] = std::max(o.color_max_nb[c], o.slot_nb[s]);
+    }
+    return o;
+}
+
+ElementSchedule build_element_schedule(const Coloration& col, int nb_elements) {
+    ElementSchedule s;
+    const int nc = col.nb_color;
+    s.color_begin.assign(nc + 1, 0);
+    for (int e = 0; e < nb_elements; ++e) s.color_begin[col.colors[e] + 1]++;
+    for (int c = 0; c < nc; ++c) s.color_begin[c + 1] += s.color_begin[c];
+    std::vector<int32_t> cur(s.color_begin.begin(), s.color_begin.end() - 1);
+    s.eids.resize(nb_elements);
+    for (int e = 0; e < nb_elements; ++e) s.eids[cur[col.colors[e]]++] = e;
+    return s;
+}
+
+}  // namespace oe

@@ -1,0 +1,388 @@
+// Host side of the GPU dynamics objects: buffers, colour schedules and the
+// launch sequence of every step.  The step sequences follow
+//   GPU_VBD::step            src/GPU/VBD/GPU_VBD.cu:38-61
+//   GPU_VBD_FEM::step        src/GPU/VBD/GPU_VBD_FEM.cu:712-756
+//   GPU_PBD::step            src/GPU/PBD/GPU_PBD.cu:20-26
+//   GPU_PBD_FEM::step        src/GPU/PBD/GPU_PBD_FEM.cu:640-717
+//   GPU_SemiExplicit::step   src/GPU/GPU_Integrator.cu:36-42
+//   GPU_FEM_Explicit::step   src/GPU/Explicit/GPU_FEM_Explicit.cu:109-115
+//   GPU_Fix_Constraint       src/GPU/GPU_Constraint.cu:94-114
+#include "oe_gpu.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+#include "oe_gather.hpp"
+#include "oe_xpbd.hpp"
+
+namespace oe {
+
+// ---------------------------------------------------------------- Engine
+Engine::Engine(int dev, size_t budget_bytes) : device(dev) {
+    OE_HIP(hipSetDevice(dev));
+    // default-priority stream (no priority other than the default anywhere)
+    OE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    pool.budget = budget_bytes;
+}
+
+Engine::~Engine() {
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+// ------------------------------------------------------ GPU_ParticleSystem
+namespace {
+std::vector<float4> to_f4(const std::vector<float>& xyz) {
+    const size_t n = xyz.size() / 3;
+    std::vector<float4> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
+    return out;
+}
+void from_f4(const std::vector<float4>& v, std::vector<float>& xyz) {
+    xyz.resize(v.size() * 3);
+    for (size_t i = 0; i < v.size(); ++i) {
+        xyz[3 * i] = v[i].x;
+        xyz[3 * i + 1] = v[i].y;
+        xyz[3 * i + 2] = v[i].z;
+    }
+}
+}  // namespace
+
+GPU_ParticleSystem::GPU_ParticleSystem(Engine* e, const std::vector<float>& xyz, const std::vector<float>& masses)
+    : eng(e), n(static_cast<int>(xyz.size() / 3)) {
+    if (static_cast<int>(masses.size()) != n) throw std::invalid_argument("mass count != vertex count");
+    const std::vector<float4> P = to_f4(xyz);
+    std::vector<float> inv(n);
+    for (int i = 0; i < n; ++i) inv[i] = 1.f / masses[i];   // w = 1/m (GPU_ParticleSystem.cu:11-12)
+    MemoryPool* pool = &eng->pool;
+    p = DBuf<float4>(pool, P);
+    last_p = DBuf<float4>(pool, P);
+    init_p = DBuf<float4>(pool, P);
+    v = DBuf<float4>(pool, std::vector<float4>(n, make_float4(0.f, 0.f, 0.f, 0.f)));
+    f = DBuf<float4>(pool, std::vector<float4>(n, make_float4(0.f, 0.f, 0.f, 0.f)));
+    m = DBuf<float>(pool, masses);
+    w = DBuf<float>(pool, inv);
+    mask = DBuf<int>(pool, std::vector<int>(n, 1));   // mask starts at 1 (GPU_ParticleSystem.cu:21)
+}
+
+PSView GPU_ParticleSystem::get_parameters() const {
+    return PSView{n, p.get(), last_p.get(), init_p.get(), v.get(), f.get(), m.get(), w.get(), mask.get()};
+}
+
+void GPU_ParticleSystem::get_position(std::vector<float>& xyz) const {
+    std::vector<float4> h;
+    p.download(h, eng->stream);
+    from_f4(h, xyz);
+}
+void GPU_ParticleSystem::get_prev_position(std::vector<float>& xyz) const {
+    std::vector<float4> h;
+    last_p.download(h, eng->stream);
+    from_f4(h, xyz);
+}
+void GPU_ParticleSystem::get_init_position(std::vector<float>& xyz) const {
+    std::vector<float4> h;
+    init_p.download(h, eng->stream);
+    from_f4(h, xyz);
+}
+void GPU_ParticleSystem::get_velocity(std::vector<float>& xyz) const {
+    std::vector<float4> h;
+    v.download(h, eng->stream);
+    from_f4(h, xyz);
+}
+void GPU_ParticleSystem::get_forces(std::vector<float>& xyz) const {
+    std::vector<float4> h;
+    f.download(h, eng->stream);
+    from_f4(h, xyz);
+}
+void GPU_ParticleSystem::get_masks(std::vector<int>& out) const { mask.download(out, eng->stream); }
+void GPU_ParticleSystem::get_mass(std::vector<float>& out) const { m.download(out, eng->stream); }
+void GPU_ParticleSystem::get_inv_mass(std::vector<float>& out) const { w.download(out, eng->stream); }
+void GPU_ParticleSystem::set_position(const std::vector<float>& xyz) { p.upload(to_f4(xyz), eng->stream); }
+void GPU_ParticleSystem::set_velocity(const std::vector<float>& xyz) { v.upload(to_f4(xyz), eng->stream); }
+void GPU_ParticleSystem::set_state(const float* pp, const float* lp, const float* vv) {
+    auto conv = [&](const float* a) {
+        std::vector<float> t(a, a + 3 * static_cast<size_t>(n));
+        return to_f4(t);
+    };
+    if (pp) p.upload(conv(pp), eng->stream);
+    if (lp) last_p.upload(conv(lp), eng->stream);
+    if (vv) v.upload(conv(vv), eng->stream);
+}
+size_t GPU_ParticleSystem::device_bytes() const {
+    return p.bytes() + last_p.bytes() + init_p.bytes() + v.bytes() + f.bytes() + m.bytes() + w.bytes() + mask.bytes();
+}
+
+// ------------------------------------------------------------ Integrators
+void GPU_Integrator::init_dynamics(GPU_ParticleSystem* ps, float dt) {
+    for (auto& d : _dynamics)
+        if (d->active) d->start(ps, dt);
+}
+void GPU_Integrator::eval_dynamics(GPU_ParticleSystem* ps, float dt) {
+    for (auto& d : _dynamics)
+        if (d->active) d->step(ps, dt);
+}
+void GPU_Integrator::eval_constraints(GPU_ParticleSystem* ps, float dt) {
+    for (auto& c : _constraints)
+        if (c->active) c->step(ps, dt);
+}
+
+void GPU_SemiExplicit::step(GPU_ParticleSystem* ps, float dt) {
+    eval_dynamics(ps, dt);
+    launch_semi_implicit(ps->get_parameters(), dt, gravity, stream);
+    if (apply_constraints) eval_constraints(ps, dt);
+}
+
+GPU_VBD::GPU_VBD(GPU_ParticleSystem* ps, int it, float rho) : iteration(it), _rho(rho) {
+    if (it < 0) throw std::invalid_argument("iteration count must be >= 0");
+    MemoryPool* pool = &ps->eng->pool;
+    stream = ps->eng->stream;
+    y.allocate(pool, ps->n);
+    OE_HIP(hipMemcpyAsync(y.get(), ps->p.get(), sizeof(float4) * ps->n, hipMemcpyDeviceToDevice, stream));
+    if (rho > 0.f) {
+        prev_it_p.allocate(pool, ps->n);
+        prev_it2_p.allocate(pool, ps->n);
+        OE_HIP(hipMemcpyAsync(prev_it_p.get(), ps->p.get(), sizeof(float4) * ps->n, hipMemcpyDeviceToDevice, stream));
+        OE_HIP(hipMemcpyAsync(prev_it2_p.get(), ps->p.get(), sizeof(float4) * ps->n, hipMemcpyDeviceToDevice, stream));
+    }
+    OE_HIP(hipStreamSynchronize(stream));
+}
+
+void GPU_VBD::add_dynamics(GPU_Dynamic* d) {
+    if (auto* fem = dynamic_cast<GPU_VBD_FEM*>(d)) fem->y = y.get();
+    GPU_Integrator::add_dynamics(d);
+}
+
+void GPU_VBD::step(GPU_ParticleSystem* ps, float dt) {
+    const PSView v = ps->get_parameters();
+    const bool cheb = _rho > 1.1920929e-7f;
+    init_dynamics(ps, dt);
+    launch_vbd_predict(v, y.get(), cheb ? prev_it_p.get() : nullptr, dt, gravity, stream);
+    float omega = 1.f;
+    for (int j = 0; j < iteration; ++j) {
+        eval_dynamics(ps, dt);
+        eval_constraints(ps, dt);
+        if (cheb) {
+            if (j == 1) omega = 2.f / (2.f - _rho * _rho);
+            else if (j > 1) omega = 4.f / (4.f - _rho * _rho * omega);
+            launch_chebyshev(v, prev_it_p.get(), prev_it2_p.get(), j, omega, stream);
+        }
+    }
+    launch_vbd_velocity(v, dt, stream);
+}
+
+void GPU_PBD::step(GPU_ParticleSystem* ps, float dt) {
+    const PSView v = ps->get_parameters();
+    launch_semi_implicit(v, dt, gravity, stream);
+    eval_dynamics(ps, dt);
+    eval_constraints(ps, dt);
+    launch_pbd_velocity(v, dt, _global_damping, stream);
+}
+
+// ---------------------------------------------------------------- GPU_FEM
+GPU_FEM::GPU_FEM(Engine* e, Element el, const HostMesh& mesh, float young, float poisson, Material material)
+    : eng(e), element(el) {
+    if (mesh.element != el) throw std::invalid_argument("mesh element type mismatch");
+    if (material != NeoHooke && material != Stable_NeoHooke)
+        throw std::invalid_argument("material not supported on the GPU path (NeoHooke / Stable_NeoHooke)");
+    FemConstants fc = build_fem_constants(mesh);
+    S = fc.S;
+    Q = fc.Q;
+    E = fc.E;
+    mt.material = material;
+    lame_parameters(young, poisson, &mt.lambda, &mt.mu);
+    mt.alpha = (material == Stable_NeoHooke) ? 1.f + mt.mu / mt.lambda : 1.f;
+    topo = DBuf<int>(&eng->pool, std::vector<int>(mesh.topo.begin(), mesh.topo.end()));
+    jxv = DBuf<float>(&eng->pool, fc.jxv);
+    dN = DBuf<float>(&eng->pool, fc.dN);
+    elem_volume = std::move(fc.elem_volume);
+}
+
+FemView GPU_FEM::fem_view() const { return FemView{E, S, Q, topo.get(), jxv.get(), dN.get()}; }
+
+GatherTile default_tile(Element e) {
+    switch (e) {
+        case Tetra: return {8, 1};
+        case Hexa: return {16, 1};
+        case Tetra10: return {8, 4};
+        case Tetra20: return {32, 1};
+        case Hexa27: return {64, 1};
+        default: return {16, 1};
+    }
+}
+
+namespace {
+void upload_owners(Engine* eng, const OwnersCSR& o, DBuf<int>& slot_vid, DBuf<int2>& slot_nbo, DBuf<int>& own) {
+    std::vector<int2> nbo(o.slot_vid.size());
+    for (size_t i = 0; i < nbo.size(); ++i) nbo[i] = make_int2(o.slot_nb[i], o.slot_off[i]);
+    slot_vid = DBuf<int>(&eng->pool, std::vector<int>(o.slot_vid.begin(), o.slot_vid.end()));
+    slot_nbo = DBuf<int2>(&eng->pool, nbo);
+    own = DBuf<int>(&eng->pool, std::vector<int>(o.own.begin(), o.own.end()));
+}
+}  // namespace
+
+// ------------------------------------------------------------ GPU_VBD_FEM
+GPU_VBD_FEM::GPU_VBD_FEM(Engine* e, Element el, const HostMesh& mesh, Material material, float young, float poisson,
+                         float damp, VBD_Version v, int target)
+    : GPU_FEM(e, el, mesh, young, poisson, material), version(v), target_color(target), damping(damp),
+      tile(default_tile(el)) {
+    const MultiGraph g = build_primal_graph(el, mesh.topo, mesh.nb_vertices());
+    if (g.n != mesh.nb_vertices()) throw std::invalid_argument("mesh has vertices used by no element");
+    colors = color_dsat(g);
+    if (version == Better_Coloration) color_reduction(g, colors, target_color);   // throws for target <= 0
+    conflicts = count_conflicts(g, colors);
+    if (conflicts != 0 && version != Better_Coloration)
+        throw std::logic_error("DSAT produced a conflicting colouring");
+    owners = build_owners(mesh, colors);
+    upload_owners(eng, owners, d_slot_vid, d_slot_nbo, d_own);
+    if (version == Better_Coloration) p_new.allocate(&eng->pool, mesh.nb_vertices());
+}
+
+GPU_VBD_FEM::~GPU_VBD_FEM() {
+    for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
+}
+
+void GPU_VBD_FEM::set_shuffle_seed(uint64_t seed) {
+    shuffle = seed != 0;
+    rng.seed(seed);
+}
+
+void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
+    if (!y) throw std::logic_error("GPU_VBD_FEM used outside a GPU_VBD integrator");
+    const int nc = colors.nb_color;
+    std::vector<int> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    if (shuffle) std::shuffle(order.begin(), order.end(), rng);
+    GatherArgs a{};
+    a.ps = ps->get_parameters();
+    a.fem = fem_view();
+    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
+    a.mt = mt;
+    a.y = y;
+    a.p_new = p_new.get();
+    a.damping = damping;
+    a.dt = dt;
+    const bool jacobi = (version == Better_Coloration);
+    a.mode = jacobi ? GATHER_VBD_JACOBI : GATHER_VBD_INPLACE;
+    hipStream_t s = eng->stream;
+    for (int c : order) {
+        a.first_slot = owners.color_begin[c];
+        a.count = owners.color_begin[c + 1] - a.first_slot;
+        if (time_gather) {
+            while (ev_pool.size() < ev_used + 2) {
+                hipEvent_t ev;
+                OE_HIP(hipEventCreate(&ev));
+                ev_pool.push_back(ev);
+            }
+            OE_HIP(hipEventRecord(ev_pool[ev_used], s));
+        }
+        if (!launch_gather(S, Q, tile.L, tile.QL, a, s)) throw std::invalid_argument("unsupported gather tile shape");
+        if (time_gather) {
+            OE_HIP(hipEventRecord(ev_pool[ev_used + 1], s));
+            ev_used += 2;
+        }
+        if (jacobi) launch_copy_colour(a.ps, d_slot_vid.get(), a.first_slot, a.count, p_new.get(), s);
+    }
+}
+
+void GPU_VBD_FEM::collect_gather_timing() {
+    if (ev_used == 0) return;
+    OE_HIP(hipStreamSynchronize(eng->stream));
+    for (size_t i = 0; i + 1 < ev_used; i += 2) {
+        float ms = 0.f;
+        OE_HIP(hipEventElapsedTime(&ms, ev_pool[i], ev_pool[i + 1]));
+        gather_ms += ms;
+        gather_launches++;
+    }
+    ev_used = 0;
+}
+
+// ------------------------------------------------------------ GPU_PBD_FEM
+GPU_PBD_FEM::GPU_PBD_FEM(Engine* e, Element el, const HostMesh& mesh, float young, float poisson, Material material,
+                         XPBD_FEM_VERSION v)
+    : GPU_FEM(e, el, mesh, young, poisson, material), version(v) {
+    const MultiGraph g = build_dual_vertex_graph(el, mesh.topo, mesh.nb_vertices());
+    colors = color_greedy_slf(g);
+    if (count_conflicts(g, colors) != 0) throw std::logic_error("element colouring has conflicts");
+    sched = build_element_schedule(colors, E);
+    d_eids = DBuf<int>(&eng->pool, std::vector<int>(sched.eids.begin(), sched.eids.end()));
+}
+
+void GPU_PBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
+    const bool v0 = (S == 4) || version == XPBD;
+    // OptiGroup launches nothing for elements other than P1 in the reference
+    // (src/GPU/PBD/GPU_PBD_FEM.cu:671-692); kept as is.
+    if (!v0 && version == OptiGroup) return;
+    XpbdArgs a{};
+    a.ps = ps->get_parameters();
+    a.fem = fem_view();
+    a.mt = mt;
+    a.eids = d_eids.get();
+    a.dt = dt;
+    for (int c = 0; c < colors.nb_color; ++c) {
+        a.first = sched.color_begin[c];
+        a.count = sched.color_begin[c + 1] - a.first;
+        if (!launch_xpbd(S, Q, v0, a, eng->stream)) throw std::invalid_argument("unsupported element for XPBD");
+    }
+}
+
+// ------------------------------------------------------- GPU_FEM_Explicit
+GPU_FEM_Explicit::GPU_FEM_Explicit(Engine* e, Element el, const HostMesh& mesh, float young, float poisson,
+                                   Material material, float damp)
+    : GPU_FEM(e, el, mesh, young, poisson, material), _damping(damp), tile(default_tile(el)),
+      n_vertices(mesh.nb_vertices()) {
+    Coloration one;
+    one.nb_color = 1;
+    one.colors.assign(n_vertices, 0);
+    const OwnersCSR o = build_owners(mesh, one);   // every vertex, ascending id
+    upload_owners(eng, o, d_slot_vid, d_slot_nbo, d_own);
+}
+
+void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
+    GatherArgs a{};
+    a.ps = ps->get_parameters();
+    a.fem = fem_view();
+    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
+    a.mt = mt;
+    a.damping = _damping;
+    a.dt = dt;
+    a.first_slot = 0;
+    a.count = n_vertices;
+    a.mode = GATHER_EXPLICIT_FORCE;
+    if (!launch_gather(S, Q, tile.L, tile.QL, a, eng->stream)) throw std::invalid_argument("unsupported gather tile");
+}
+
+// ------------------------------------------------------------ constraints
+GPU_Fix_Constraint::GPU_Fix_Constraint(Engine* e, const std::vector<float>& xyz, const std::vector<int>& sel)
+    : ids(sel), stream(e->stream) {
+    float com[3] = {0.f, 0.f, 0.f};
+    for (int i : ids) {
+        if (i < 0 || static_cast<size_t>(i) >= xyz.size() / 3) throw std::invalid_argument("fixed id out of range");
+        com[0] += xyz[3 * i]; com[1] += xyz[3 * i + 1]; com[2] += xyz[3 * i + 2];
+    }
+    const float cnt = static_cast<float>(ids.size());
+    for (int d = 0; d < 3; ++d) {
+        t.com[d] = ids.empty() ? 0.f : com[d] / cnt;
+        t.offset[d] = 0.f;
+    }
+    const float I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    std::memcpy(t.R, I, sizeof(I));
+    d_ids = DBuf<int>(&e->pool, ids);
+}
+
+void GPU_Fix_Constraint::step(GPU_ParticleSystem* ps, float) {
+    launch_fix(ps->get_parameters(), d_ids.get(), static_cast<int>(ids.size()), t, stream);
+}
+
+GPU_Box_Limit::GPU_Box_Limit(hipStream_t s, const float pmin_[3], const float pmax_[3]) : stream(s) {
+    for (int d = 0; d < 3; ++d) { pmin[d] = pmin_[d]; pmax[d] = pmax_[d]; }
+}
+
+void GPU_Box_Limit::step(GPU_ParticleSystem* ps, float) {
+    launch_box(ps->get_parameters(), pmin, pmax, stream);
+}
+
+}  // namespace oe

@@ -1,0 +1,201 @@
+// GPU dynamics object model.  It keeps the reference's plugin interface for the
+// GPU soft-body path so that the same composition works:
+//   GPU_ParticleSystem        include/GPU/GPU_ParticleSystem.h:54-73
+//   GPU_Dynamic               include/GPU/GPU_Dynamic.h:6-11   (start / step / active)
+//   GPU_Integrator            include/GPU/GPU_Integrator.h:11-32 (dynamics + constraints)
+//   GPU_SemiExplicit          include/GPU/GPU_Integrator.h:34-36
+//   GPU_VBD                   include/GPU/VBD/GPU_VBD.h:13-48
+//   GPU_PBD                   include/GPU/PBD/GPU_PBD.h
+//   GPU_FEM                   include/GPU/GPU_FEM.h:61-101
+//   GPU_VBD_FEM               include/GPU/VBD/GPU_VBD_FEM.h:62-130
+//   GPU_PBD_FEM               include/GPU/PBD/GPU_PBD_FEM.h:15-51
+//   GPU_FEM_Explicit          include/GPU/Explicit/GPU_FEM_Explicit.h:12-33
+//   GPU_Fix_Constraint        include/GPU/GPU_Constraint.h:11-18
+//   GPU_Box_Limit             include/GPU/GPU_Constraint.h:20-24
+// The state lives in SoA device arrays owned by the engine's memory pool and
+// every launch goes to the engine's stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "oe_device.hpp"
+#include "oe_elementwise.hpp"
+#include "oe_host.hpp"
+
+namespace oe {
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    MemoryPool pool;
+    explicit Engine(int dev, size_t budget_bytes = SIZE_MAX);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+};
+
+struct GPU_ParticleSystem {
+    GPU_ParticleSystem(Engine* eng, const std::vector<float>& xyz, const std::vector<float>& masses);
+    int nb_particles() const { return n; }
+    PSView get_parameters() const;
+    void get_position(std::vector<float>& xyz) const;
+    void get_prev_position(std::vector<float>& xyz) const;
+    void get_init_position(std::vector<float>& xyz) const;
+    void get_velocity(std::vector<float>& xyz) const;
+    void get_forces(std::vector<float>& xyz) const;
+    void get_masks(std::vector<int>& m) const;
+    void get_mass(std::vector<float>& m) const;
+    void get_inv_mass(std::vector<float>& w) const;
+    void set_position(const std::vector<float>& xyz);
+    void set_velocity(const std::vector<float>& xyz);
+    void set_state(const float* p, const float* last_p, const float* v);
+    size_t device_bytes() const;
+
+    Engine* eng;
+    int n;
+    DBuf<float4> p, last_p, init_p, v, f;
+    DBuf<float> m, w;
+    DBuf<int> mask;
+};
+
+struct GPU_Dynamic {
+    GPU_Dynamic() = default;
+    virtual ~GPU_Dynamic() = default;
+    virtual void start(GPU_ParticleSystem*, float) {}            // once per step
+    virtual void step(GPU_ParticleSystem* ps, float dt) = 0;      // once per iteration / substep
+    bool active = true;
+};
+
+struct GPU_Integrator : GPU_Dynamic {
+    explicit GPU_Integrator(bool apply = true) : apply_constraints(apply) {}
+    virtual void add_dynamics(GPU_Dynamic* d) { _dynamics.emplace_back(d); }
+    virtual void add_constraint(GPU_Dynamic* c) { _constraints.emplace_back(c); }
+    virtual void init_dynamics(GPU_ParticleSystem* ps, float dt);
+    virtual void eval_dynamics(GPU_ParticleSystem* ps, float dt);
+    virtual void eval_constraints(GPU_ParticleSystem* ps, float dt);
+
+    bool apply_constraints;
+    float gravity[3] = {0.f, -9.81f, 0.f};
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<GPU_Dynamic>> _dynamics;
+    std::vector<std::unique_ptr<GPU_Dynamic>> _constraints;
+};
+
+struct GPU_SemiExplicit final : GPU_Integrator {
+    void step(GPU_ParticleSystem* ps, float dt) override;
+};
+
+struct GPU_VBD final : GPU_Integrator {
+    GPU_VBD(GPU_ParticleSystem* ps, int it, float rho);
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    void add_dynamics(GPU_Dynamic* d) override;
+    int iteration;
+    float _rho;
+    DBuf<float4> y, prev_it_p, prev_it2_p;
+};
+
+struct GPU_PBD final : GPU_Integrator {
+    explicit GPU_PBD(float global_damping = 0.f) : _global_damping(global_damping) {}
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    float _global_damping;
+};
+
+struct GPU_FEM : GPU_Dynamic {
+    GPU_FEM(Engine* eng, Element element, const HostMesh& mesh, float young, float poisson, Material material);
+    FemView fem_view() const;
+    Engine* eng;
+    Element element;
+    int S, Q, E;
+    MaterialView mt;
+    DBuf<int> topo;
+    DBuf<float> jxv, dN;
+    std::vector<double> elem_volume;
+};
+
+enum VBD_Version : int { Base = 0, Threaded_Quadrature = 1, Reduction_Symmetry = 2, Better_Coloration = 3 };
+
+// Tile shape of the per-vertex gather: lanes per vertex, quadrature points per lane.
+struct GatherTile {
+    int L = 16;
+    int QL = 1;
+};
+GatherTile default_tile(Element e);
+
+struct GPU_VBD_FEM final : GPU_FEM {
+    GPU_VBD_FEM(Engine* eng, Element element, const HostMesh& mesh, Material material, float young, float poisson,
+                float damping, VBD_Version v = Reduction_Symmetry, int target_color = -1);
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    // colour order of the next sweep: ascending, or shuffled from a seed
+    void set_shuffle_seed(uint64_t seed);
+    int nb_colors() const { return colors.nb_color; }
+
+    Coloration colors;
+    OwnersCSR owners;   // host copy (colour ranges are read at launch time)
+    VBD_Version version;
+    int target_color;
+    float damping;
+    GatherTile tile;
+    bool shuffle = false;
+    std::mt19937_64 rng;
+    const float4* y = nullptr;   // bound by GPU_VBD::add_dynamics
+    DBuf<int> d_slot_vid;
+    DBuf<int2> d_slot_nbo;
+    DBuf<int> d_own;
+    DBuf<float4> p_new;
+    int64_t conflicts = 0;
+    // optional per-launch timing of the gather kernel (events around each colour)
+    bool time_gather = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double gather_ms = 0.0;
+    int64_t gather_launches = 0;
+    void collect_gather_timing();
+    ~GPU_VBD_FEM() override;
+};
+
+enum XPBD_FEM_VERSION : int { XPBD = 0, Quadrature = 1, OptiShared = 2, OptiGroup = 3 };
+
+struct GPU_PBD_FEM final : GPU_FEM {
+    GPU_PBD_FEM(Engine* eng, Element element, const HostMesh& mesh, float young, float poisson, Material material,
+                XPBD_FEM_VERSION v = OptiShared);
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    Coloration colors;   // element colours
+    ElementSchedule sched;
+    XPBD_FEM_VERSION version;
+    DBuf<int> d_eids;
+};
+
+struct GPU_FEM_Explicit final : GPU_FEM {
+    GPU_FEM_Explicit(Engine* eng, Element element, const HostMesh& mesh, float young, float poisson,
+                     Material material, float damping);
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    float _damping;
+    GatherTile tile;
+    int n_vertices;
+    DBuf<int> d_slot_vid;
+    DBuf<int2> d_slot_nbo;
+    DBuf<int> d_own;
+};
+
+struct GPU_Fix_Constraint final : GPU_Dynamic {
+    GPU_Fix_Constraint(Engine* eng, const std::vector<float>& xyz, const std::vector<int>& ids);
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    FixTransform t;
+    std::vector<int> ids;
+    DBuf<int> d_ids;
+    hipStream_t stream;
+};
+
+struct GPU_Box_Limit final : GPU_Dynamic {
+    GPU_Box_Limit(hipStream_t s, const float pmin_[3], const float pmax_[3]);
+    void step(GPU_ParticleSystem* ps, float dt) override;
+    float pmin[3], pmax[3];
+    hipStream_t stream;
+};
+
+}  // namespace oe

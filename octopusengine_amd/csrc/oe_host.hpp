@@ -1,0 +1,137 @@
+// Host-side (CPU) part of the engine: element shapes, beam meshes and their
+// high-order conversions, graph colouring, per-(element, quadrature) constants,
+// masses and the colour-major owners CSR that the HIP kernels consume.
+//
+// Everything here runs once per scene at set-up time; nothing here is on the
+// per-step path.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace oe {
+
+// Element ids keep the integer values of the reference enum
+// (octopus/include/Mesh/Elements.h:4-15) so that a caller can pass the
+// reference's own enum values through the C ABI unchanged.
+enum Element : int {
+    Line = 0, Triangle = 1, Quad = 2,
+    Tetra = 3, Pyramid = 4, Prism = 5, Hexa = 6,
+    Tetra10 = 7, Tetra20 = 8, Hexa27 = 9
+};
+
+// Material ids keep the reference enum values
+// (octopus/include/Dynamic/FEM/ContinuousMaterial.h:8-10).
+enum Material : int { Hooke = 0, StVK = 1, NeoHooke = 2, Stable_NeoHooke = 3 };
+
+// Mass distribution, reference octopus/include/Dynamic/FEM/FEM_Shape.h:693-696.
+enum Mass_Distribution : int { Uniform = 0, FemShape = 1 };
+
+bool element_supported(int e);
+int elem_nb_vertices(Element e);
+int elem_nb_quadrature(Element e);
+std::string element_name(Element e);
+
+// Reference element: node coordinates, quadrature rule and the shape-function
+// gradients at every quadrature point.
+struct ShapeTable {
+    Element element;
+    int S = 0;                        // nodes per element
+    int Q = 0;                        // quadrature points
+    std::vector<double> nodes;        // S x 3 reference coordinates
+    std::vector<double> qpoints;      // Q x 3
+    std::vector<double> weights;      // Q
+    std::vector<double> dN;           // Q x S x 3 : dN[(q*S + k)*3 + c]
+    std::vector<double> coeff;        // basis coefficients (monomial x node)
+    std::vector<int> mono;            // monomial exponents, S x 3
+    // Shape function values N_k at an arbitrary point (used by tests).
+    void eval(const double* x, double* N, double* dNdx) const;
+};
+
+const ShapeTable& shape_table(Element e);
+
+// A mesh with one element type.  xyz is float, as in the reference geometry
+// (std::vector<glm::vec3>).
+struct HostMesh {
+    Element element = Tetra;
+    std::vector<float> xyz;     // 3 * N
+    std::vector<int32_t> topo;  // S * E
+    int nb_vertices() const { return static_cast<int>(xyz.size() / 3); }
+    int nb_elements() const { return static_cast<int>(topo.size() / elem_nb_vertices(element)); }
+};
+
+// Structured beam generators (reference src/Mesh/Generator/BeamGenerator.cpp:5-48,86-92)
+// and high-order conversions (:172-377).  `element` is the final element type:
+// Tetra/Tetra10/Tetra20 use the 6-tetrahedra split of every cell, Hexa/Hexa27
+// use one hexahedron per cell.
+HostMesh beam_mesh(Element element, int nx, int ny, int nz, float sx, float sy, float sz);
+void tetra4_to_tetra10(HostMesh& m);
+void tetra4_to_tetra20(HostMesh& m);
+void hexa_to_hexa27(HostMesh& m);
+
+// ---------------------------------------------------------------- colouring
+// Adjacency with multiplicity: every co-element pair is inserted once per
+// shared element, exactly like the reference's std::multiset adjacency
+// (include/Tools/Graph.h:36-54).  Stored as CSR, neighbours sorted ascending,
+// duplicates kept adjacent.
+struct MultiGraph {
+    int n = 0;
+    std::vector<int64_t> off;   // n + 1
+    std::vector<int32_t> adj;   // sorted per row, with repeats
+    int degree(int v) const { return static_cast<int>(off[v + 1] - off[v]); }
+};
+
+MultiGraph build_primal_graph(Element e, const std::vector<int32_t>& topo, int nb_vertices);
+MultiGraph build_dual_vertex_graph(Element e, const std::vector<int32_t>& topo, int nb_vertices);
+
+struct Coloration {
+    int nb_color = 0;
+    std::vector<int32_t> colors;
+};
+
+Coloration color_dsat(const MultiGraph& g);                       // Graph.h:321-371
+void color_reduction(const MultiGraph& g, Coloration& c, int target);  // Graph.h:941-969
+Coloration color_greedy_slf(const MultiGraph& g);                 // Graph.h:237-277
+int64_t count_conflicts(const MultiGraph& g, const Coloration& c);  // Graph.h:863-881 (sum)
+
+// ------------------------------------------------------------ FEM constants
+// Per (element, quadrature point): JX_inv (column-major 3x3) and V = |det J| w
+// (reference src/GPU/GPU_FEM.cu:176-214).  Packed as 10 floats per record.
+struct FemConstants {
+    int S = 0, Q = 0, E = 0;
+    std::vector<float> jxv;     // E * Q * 10
+    std::vector<float> dN;      // Q * S * 3 (float)
+    std::vector<double> elem_volume;  // E
+};
+FemConstants build_fem_constants(const HostMesh& m);
+
+// Reference src/Dynamic/FEM/FEM_Shape.cpp:86-121.
+std::vector<float> compute_fem_mass(const HostMesh& m, const FemConstants& fc, float density,
+                                    Mass_Distribution distrib);
+
+// Lame parameters, reference src/GPU/GPU_FEM.cu:162-163 (float arithmetic).
+void lame_parameters(float young, float poisson, float* lambda, float* mu);
+
+// Colour-major owners CSR (reference src/GPU/VBD/GPU_VBD_FEM.cu:594-690):
+// slots are the vertices sorted by colour (ascending id inside a colour); each
+// slot lists its incident elements in ascending element id, packed as
+// (eid << 5) | local_node.
+struct OwnersCSR {
+    std::vector<int32_t> slot_vid;      // N
+    std::vector<int32_t> slot_nb;       // N  incident element count
+    std::vector<int32_t> slot_off;      // N  offset into own
+    std::vector<int32_t> own;           // E * S packed (eid << 5 | ref)
+    std::vector<int32_t> color_begin;   // nb_color + 1 (slot ranges)
+    std::vector<int32_t> color_max_nb;  // max incident elements per colour
+};
+OwnersCSR build_owners(const HostMesh& m, const Coloration& col);
+
+// Element colour schedule for XPBD (reference src/GPU/PBD/GPU_PBD_FEM.cu:528-553).
+struct ElementSchedule {
+    std::vector<int32_t> eids;          // E, colour-major, ascending inside a colour
+    std::vector<int32_t> color_begin;   // nb_color + 1
+};
+ElementSchedule build_element_schedule(const Coloration& col, int nb_elements);
+
+}  // namespace oe

@@ -1,0 +1,182 @@
+// Device code shared by the VBD, XPBD and explicit paths: the per-quadrature
+// kinematics (Jx, F, dF/dx), the hyperelastic stress and the per-vertex
+// Hessian block, cross-lane sums, and the FP64 3x3 solve.
+//
+// Conventions: 3x3 matrices are float[9] in column-major order (m[c*3 + r]),
+// as the reference's glm matrices are, so JX_inv records built on the host can
+// be read directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "oe_device.hpp"
+
+namespace oe {
+
+// Per-vertex accumulator: force (3) + lower triangle of the symmetric 3x3
+// Hessian block (H00, H10, H20, H11, H21, H22).
+struct Acc9 {
+    float v[9];
+};
+
+__device__ __forceinline__ void acc_zero(Acc9& a) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.v[i] = 0.f;
+}
+
+// Sum over groups of L consecutive lanes (L a power of two, L <= 64).
+template <int L>
+__device__ __forceinline__ float group_sum(float x) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, L);
+    return x;
+}
+
+template <int L>
+__device__ __forceinline__ void group_sum(Acc9& a) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.v[i] = group_sum<L>(a.v[i]);
+}
+
+// Jx = sum_k x_k (x) dN_k  (column-major: Jx[c*3+r] = sum_k x_k[r] dN_k[c])
+template <int S>
+__device__ __forceinline__ void jx_from_nodes(const float (&xs)[S][3], const float* __restrict__ dn, float Jx[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Jx[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const float d0 = dn[3 * k], d1 = dn[3 * k + 1], d2 = dn[3 * k + 2];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            Jx[0 + r] = fmaf(xs[k][r], d0, Jx[0 + r]);
+            Jx[3 + r] = fmaf(xs[k][r], d1, Jx[3 + r]);
+            Jx[6 + r] = fmaf(xs[k][r], d2, Jx[6 + r]);
+        }
+    }
+}
+
+// F = Jx * JX_inv (both column-major)
+__device__ __forceinline__ void mat_mul(const float A[9], const float B[9], float C[9]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            C[c * 3 + r] = A[0 * 3 + r] * B[c * 3 + 0] + A[1 * 3 + r] * B[c * 3 + 1] + A[2 * 3 + r] * B[c * 3 + 2];
+}
+
+// cofactor matrix: column c = cross of the two other columns (cyclic)
+__device__ __forceinline__ void cofactor(const float F[9], float C[9]) {
+    const float* a = F;       // column 0
+    const float* b = F + 3;   // column 1
+    const float* c = F + 6;   // column 2
+    C[0] = b[1] * c[2] - b[2] * c[1];
+    C[1] = b[2] * c[0] - b[0] * c[2];
+    C[2] = b[0] * c[1] - b[1] * c[0];
+    C[3] = c[1] * a[2] - c[2] * a[1];
+    C[4] = c[2] * a[0] - c[0] * a[2];
+    C[5] = c[0] * a[1] - c[1] * a[0];
+    C[6] = a[1] * b[2] - a[2] * b[1];
+    C[7] = a[2] * b[0] - a[0] * b[2];
+    C[8] = a[0] * b[1] - a[1] * b[0];
+}
+
+// det F = column0 . cof column0.  Elements close to inversion (J < 0.1) get
+// the determinant re-evaluated in double precision.
+__device__ __forceinline__ float det_from_cof(const float F[9], const float C[9]) {
+    float J = F[0] * C[0] + F[1] * C[1] + F[2] * C[2];
+    if (J < 0.1f) {
+        const double a0 = F[0], a1 = F[1], a2 = F[2], b0 = F[3], b1 = F[4], b2 = F[5], c0 = F[6], c1 = F[7], c2 = F[8];
+        const double d = a0 * (b1 * c2 - b2 * c1) + a1 * (b2 * c0 - b0 * c2) + a2 * (b0 * c1 - b1 * c0);
+        J = static_cast<float>(d);
+    }
+    return J;
+}
+
+// One (element, quadrature point) contribution to the force on node r and to
+// its 3x3 Hessian block, for the Neo-Hookean family used by the reference's
+// GPU VBD / explicit paths (src/GPU/GPU_FEM_Material.cu:25-36,167-207,250-261):
+//   P = mu F + lambda (J - alpha) cof F
+//   f_r -= V P g,   g = JX_inv^T dN_r
+//   H_r += V (mu |g|^2 I + lambda c c^T),   c = cof F g
+// The reference assembles H_r from six 3x3 blocks of d2W/dF2; the
+// lambda (J - alpha) d2J/dF2 blocks are antisymmetric and cancel in the
+// symmetric assembly, which leaves the closed form above.
+__device__ __forceinline__ void nh_contrib(const float Jx[9], const float* __restrict__ rec, const float* __restrict__ dnr,
+                                           const MaterialView& mt, Acc9& a) {
+    float JXi[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
+    const float V = rec[9];
+    float F[9];
+    mat_mul(Jx, JXi, F);
+    // g_c = sum_l JX_inv(l, c) dN_r[l]
+    float g[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[c] = JXi[c * 3 + 0] * dnr[0] + JXi[c * 3 + 1] * dnr[1] + JXi[c * 3 + 2] * dnr[2];
+    float C[9];
+    cofactor(F, C);
+    const float J = det_from_cof(F, C);
+    const float s = mt.lambda * (J - mt.alpha);
+    // P g = mu F g + s C g ; c = C g
+    float cg[3], Fg[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        cg[r] = C[r] * g[0] + C[3 + r] * g[1] + C[6 + r] * g[2];
+        Fg[r] = F[r] * g[0] + F[3 + r] * g[1] + F[6 + r] * g[2];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) a.v[r] -= V * (mt.mu * Fg[r] + s * cg[r]);
+    const float gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    const float d = V * mt.mu * gg;
+    const float l = V * mt.lambda;
+    a.v[3] += d + l * cg[0] * cg[0];
+    a.v[4] += l * cg[1] * cg[0];
+    a.v[5] += l * cg[2] * cg[0];
+    a.v[6] += d + l * cg[1] * cg[1];
+    a.v[7] += l * cg[2] * cg[1];
+    a.v[8] += d + l * cg[2] * cg[2];
+}
+
+// Newton step of the local VBD problem in double precision
+// (reference compute_correction, src/GPU/VBD/GPU_VBD_FEM.cu:107-122):
+//   f -= (k/dt) H (p - last_p);  H += (k/dt) H
+//   f -= (m/dt^2)(p - y);        H += (m/dt^2) I
+//   dx = |det H| > 1e-6 ? H^-1 f : 0
+__device__ __forceinline__ void vbd_local_solve(const Acc9& a, float4 p, float4 lp, float4 y, float m, float damping,
+                                                float dt, double dx[3]) {
+    double f0 = a.v[0], f1 = a.v[1], f2 = a.v[2];
+    double h00 = a.v[3], h10 = a.v[4], h20 = a.v[5], h11 = a.v[6], h21 = a.v[7], h22 = a.v[8];
+    const double ddt = dt;
+    if (damping > 0.f) {
+        const double k = static_cast<double>(damping) / ddt;
+        const double u0 = static_cast<double>(p.x) - lp.x, u1 = static_cast<double>(p.y) - lp.y,
+                     u2 = static_cast<double>(p.z) - lp.z;
+        f0 -= k * (h00 * u0 + h10 * u1 + h20 * u2);
+        f1 -= k * (h10 * u0 + h11 * u1 + h21 * u2);
+        f2 -= k * (h20 * u0 + h21 * u1 + h22 * u2);
+        const double s = 1.0 + k;
+        h00 *= s; h10 *= s; h20 *= s; h11 *= s; h21 *= s; h22 *= s;
+    }
+    const double mh2 = static_cast<double>(m) / (ddt * ddt);
+    f0 -= mh2 * (static_cast<double>(p.x) - y.x);
+    f1 -= mh2 * (static_cast<double>(p.y) - y.y);
+    f2 -= mh2 * (static_cast<double>(p.z) - y.z);
+    h00 += mh2; h11 += mh2; h22 += mh2;
+    const double c00 = h11 * h22 - h21 * h21;
+    const double c01 = h21 * h20 - h10 * h22;
+    const double c02 = h10 * h21 - h11 * h20;
+    const double det = h00 * c00 + h10 * c01 + h20 * c02;
+    if (fabs(det) > 1e-6) {
+        const double c11 = h00 * h22 - h20 * h20;
+        const double c12 = h20 * h10 - h00 * h21;
+        const double c22 = h00 * h11 - h10 * h10;
+        const double inv = 1.0 / det;
+        dx[0] = (c00 * f0 + c01 * f1 + c02 * f2) * inv;
+        dx[1] = (c01 * f0 + c11 * f1 + c12 * f2) * inv;
+        dx[2] = (c02 * f0 + c12 * f1 + c22 * f2) * inv;
+    } else {
+        dx[0] = dx[1] = dx[2] = 0.0;
+    }
+}
+
+}  // namespace oe

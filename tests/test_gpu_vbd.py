@@ -1,0 +1,106 @@
+"""VBD on the GPU vs the FP64 oracle (same scene, same colour order).
+
+Tolerance (from the path's accuracy target): max vertex-position error
+<= 1e-5 of the bounding-box diagonal after 1 step, <= 1e-3 after 100 steps.
+"""
+import numpy as np
+import pytest
+
+from conftest import beam_diag
+from scenes import make_pair, max_err
+
+pytestmark = pytest.mark.gpu
+
+DIAG = beam_diag()
+TOL1 = 1e-5 * DIAG
+TOL100 = 1e-3 * DIAG
+
+# (element, cells) small enough for the oracle to run in seconds
+CASES = [
+    (3, (16, 4, 4)),    # P1, the 425-vertex beam
+    (7, (8, 2, 2)),     # P2
+    (6, (8, 2, 2)),     # Q1
+    (9, (4, 2, 2)),     # Q2
+    (8, (4, 1, 1)),     # P3
+]
+
+
+def _ids(case):
+    return f"el{case[0]}-{case[1][0]}x{case[1][1]}x{case[1][2]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+def test_vbd_one_step(engine, case):
+    el, cells = case
+    g, o, rest, _ = make_pair(engine, el, cells, material=3, young=5e5, poisson=0.49)
+    g.step(0.005, 1)
+    o.step(0.005, 1)
+    err = max_err(g.positions(), o.positions())
+    print(f"VBD el={el} cells={cells} 1-step err={err:.3e} (tol {TOL1:.3e})", flush=True)
+    assert err <= TOL1
+    # fixed vertices snapped back and masked
+    ids = np.nonzero(rest[:, 0] < 1e-6)[0]
+    assert np.all(g.masks()[ids] == 0)
+    assert max_err(g.positions()[ids], rest[ids]) <= 1e-6
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=[_ids(c) for c in CASES[:3]])
+def test_vbd_many_steps(engine, case):
+    el, cells = case
+    g, o, rest, _ = make_pair(engine, el, cells, material=3, young=5e5, poisson=0.49)
+    steps = 100 if el in (3, 6) else 40
+    for _ in range(steps):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    err = max_err(g.positions(), o.positions())
+    disp = float(np.abs(o.positions() - rest).max())
+    print(f"VBD el={el} {steps} steps err={err:.3e} tip displacement={disp:.3e}", flush=True)
+    assert disp > 1e-3          # the beam really moved
+    assert err <= TOL100
+
+
+def test_vbd_neohooke_and_damping(engine):
+    g, o, _, _ = make_pair(engine, 7, (6, 2, 2), material=2, young=1e6, poisson=0.45, damping=1e-3)
+    for _ in range(5):
+        g.step(0.005, 2)
+        o.step(0.005, 2)
+    err = max_err(g.positions(), o.positions())
+    print(f"VBD NeoHooke damped err={err:.3e}", flush=True)
+    assert err <= TOL1 * 5
+
+
+def test_vbd_tiles_agree(engine):
+    """Every gather tile shape computes the same step (up to FP32 summation order)."""
+    ref = None
+    for tile in [(4, 1), (8, 1), (16, 1), (64, 1), (4, 2), (8, 4)]:
+        g, o, _, _ = make_pair(engine, 7, (6, 2, 2), tile=tile)
+        g.step(0.005, 1)
+        p = g.positions()
+        if ref is None:
+            o.step(0.005, 1)
+            ref = o.positions()
+        err = max_err(p, ref)
+        print(f"tile {tile} err {err:.3e}", flush=True)
+        assert err <= TOL1
+
+
+def test_vbd_graph_replay_matches_eager(engine):
+    a, _, _, _ = make_pair(engine, 9, (4, 2, 2))
+    b, _, _, _ = make_pair(engine, 9, (4, 2, 2))
+    b.set_graph(True)
+    for i in range(4):
+        a.step(0.005, 1)
+        b.step(0.005, 1)
+        engine.synchronize()
+        assert max_err(a.positions(), b.positions()) == 0.0, f"replay {i}"
+
+
+def test_vbd_free_fall_closed_form(engine):
+    """lambda = mu ~ 0 (E tiny): every free vertex follows x + g dt^2 per step from rest."""
+    g, o, rest, _ = make_pair(engine, 7, (4, 1, 1), young=1e-6, poisson=0.3, fix=False, box=False)
+    dt = 0.01
+    g.step(dt, 1)
+    p = g.positions()
+    expect = rest.astype(np.float64).copy()
+    expect[:, 1] += -9.81 * dt * dt
+    assert max_err(p, expect) <= 1e-6
