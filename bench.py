@@ -1,17 +1,27 @@
-"""Benchmark: VBD vertex-updates/s and substep latency on one MI355X.
+"""Benchmark: VBD vertex-updates/s and substep latency on one MI355X,
+batched vs plain.
 
-Default workload (N=1): plain VBD, one scene - the 1.0 x 0.25 x 0.25 m
-cantilever beam as 64x16x16 hexes split into quadratic P2 tetrahedra
-(140,481 vertices, 98,304 elements), Stable Neo-Hookean E=5e5 Pa, nu=0.49,
-x=0 face fixed, gravity, dt=0.005 s, 10 VBD iterations per step, FP32 state
-with FP64 3x3 solves, DSAT colouring, ascending colour order.
+Headline workload (N = 1 GPU): batched VBD - 256 independent cantilever beams
+(1.0 x 0.25 x 0.25 m) whose grid sizes are drawn from a fixed seed
+(nx ~ U{8..64}, ny, nz ~ U{4..16}), each a P2-tetrahedron or Q2-hexahedron
+mesh with probability 1/2, Stable Neo-Hookean with E ~ logU[1e5, 1e7] Pa and
+nu ~ U[0.3, 0.49], x = 0 face fixed, gravity, dt = 0.005 s, 10 VBD
+iterations per step, FP32 state and FP64 3x3 solves.  One step of the batch
+advances every scene by one step; all scenes of one global colour are solved
+by one launch per element type.
 
-A step is one call of step(dt, 1): predictor, 10 iterations of (one gather
-launch per colour + fixation + box limit), velocity update.  One
-vertex-update = one vertex's local 3x3 solve in one iteration; throughput =
-N_vertices * iterations * steps / wall time of the timed region.
+Also measured in the same run:
+  * the plain per-scene loop on the same 256 scenes (each scene stepped on its
+    own, one launch per colour) - the "plain" form of the metric;
+  * the plain single-scene P2 64x16x16 beam (140,481 vertices, E = 5e5,
+    nu = 0.49);
+  * the CPU baseline: the FP64 oracle (single thread) on a bounded sample of
+    the single-scene workload, and the reference CPU XPBD path on its own
+    425-vertex beam.
 
-Prints one JSON line (see README of the harness contract).
+A vertex-update is one vertex's local 3x3 solve in one iteration; throughput =
+sum over scenes of N * iterations * steps / wall time of the timed region.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -19,9 +29,9 @@ import argparse
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -31,6 +41,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 SIZE = (1.0, 0.25, 0.25)
+DT = 0.005
+ITERS = 10
 
 
 def algorithmic_bytes_per_iteration(n_vertices: int, n_elements: int, S: int, Q: int) -> float:
@@ -53,161 +65,243 @@ def cpu_info() -> tuple[str, int]:
     return model, os.cpu_count() or 0
 
 
-def run_cpu_baseline(xyz, topo, steps: int = 2) -> dict:
-    """FP64 oracle VBD (single thread) on a bounded sample of the same workload."""
+def config4_specs(n: int = 256, seed: int = 4):
+    """(element, cells, E, nu) of the batched workload, drawn from `seed`."""
+    import octopusengine_amd as oe
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nx = int(rng.integers(8, 65))
+        ny = int(rng.integers(4, 17))
+        nz = int(rng.integers(4, 17))
+        el = oe.TETRA10 if rng.random() < 0.5 else oe.HEXA27
+        E = float(10.0 ** rng.uniform(5.0, 7.0))
+        nu = float(rng.uniform(0.3, 0.49))
+        specs.append((el, (nx, ny, nz), E, nu))
+    return specs
+
+
+def build_scenes(eng, specs, workers: int):
+    import octopusengine_amd as oe
+
+    def one(sp):
+        el, cells, E, nu = sp
+        return oe.make_scene(eng, el, cells, SIZE, young=E, poisson=nu, iterations=ITERS)
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, specs))
+
+
+def timed(fn, steps, sync):
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def batched_leg(args, eng) -> dict:
+    import octopusengine_amd as oe
+
+    specs = config4_specs(args.scenes, args.seed)
+    t0 = time.perf_counter()
+    scenes = build_scenes(eng, specs, args.workers)
+    setup_s = time.perf_counter() - t0
+    nv = int(sum(s.n_vertices for s in scenes))
+    n_p2 = sum(1 for s in scenes if s.element == oe.TETRA10)
+    b = oe.Batch(eng, scenes)
+    info = b.info()
+    vu_step = info["vertex_updates"]
+    for _ in range(args.warmup):
+        b.step(DT, 1)
+    # timed region: eager launches, HIP events around every gather launch
+    b.set_timing(True)
+    b.get_timing()
+    wall = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
+    gather_ms, launches = b.get_timing()
+    b.set_timing(False)
+    # same steps from a captured HIP graph
+    b.set_graph(True)
+    b.step(DT, 1)
+    wall_graph = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        b.step(DT, 1)
+        eng.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    b.set_graph(False)
+    # plain per-scene loop on the same scenes (each scene steps on its own)
+    plain_steps = max(1, args.plain_steps)
+
+    def plain_step():
+        for s in scenes:
+            s.step(DT, 1)
+
+    plain_step()
+    wall_plain = timed(plain_step, plain_steps, eng.synchronize)
+    finite = all(bool(np.isfinite(s.positions()).all()) for s in scenes[:8])
+    bytes_total = info["gather_bytes"] * args.steps
+    avg_launch_s = gather_ms / 1e3 / max(launches, 1)
+    achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
+    b.close()
+    return {
+        "value": vu_step * args.steps / wall,
+        "graph_value": vu_step * args.steps / wall_graph,
+        "ms_per_step": wall / args.steps * 1e3,
+        "graph_ms_per_step": wall_graph / args.steps * 1e3,
+        "p50_substep_ms": float(np.percentile(lat, 50)),
+        "p99_substep_ms": float(np.percentile(lat, 99)),
+        "plain_value": vu_step * plain_steps / wall_plain,
+        "plain_ms_per_step": wall_plain / plain_steps * 1e3,
+        "speedup_vs_plain": (vu_step * args.steps / wall_graph) / (vu_step * plain_steps / wall_plain),
+        "scenes": len(scenes), "p2_scenes": n_p2, "vertices": nv, "vertex_updates_per_step": vu_step,
+        "gather_launches_per_step": info["gather_launches"], "setup_s": setup_s, "positions_finite": finite,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_gather (VBD colour pass, all scenes)",
+                     "bytes_per_launch": bytes_total / max(launches, 1), "avg_launch_us": avg_launch_s * 1e6,
+                     "launches": launches},
+        "_scenes": scenes,
+    }
+
+
+def plain_single_leg(args, eng) -> dict:
+    import octopusengine_amd as oe
+
+    sc = oe.make_scene(eng, oe.TETRA10, (64, 16, 16), SIZE, young=5e5, poisson=0.49, iterations=ITERS)
+    N, E, S, Q = sc.n_vertices, sc.n_elements, sc.S, sc.Q
+    for _ in range(args.warmup):
+        sc.step(DT, 1)
+    sc.set_timing(True)
+    sc.get_timing()
+    wall = timed(lambda: sc.step(DT, 1), args.steps, eng.synchronize)
+    gather_ms, launches = sc.get_timing()
+    sc.set_timing(False)
+    sc.set_graph(True)
+    sc.step(DT, 1)
+    wall_graph = timed(lambda: sc.step(DT, 1), args.steps, eng.synchronize)
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        sc.step(DT, 1)
+        eng.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    sc.set_graph(False)
+    bytes_per_launch = algorithmic_bytes_per_iteration(N, E, S, Q) * ITERS * args.steps / max(launches, 1)
+    avg_launch_s = gather_ms / 1e3 / max(launches, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    out = {
+        "value": N * ITERS * args.steps / wall,
+        "graph_value": N * ITERS * args.steps / wall_graph,
+        "ms_per_step": wall / args.steps * 1e3,
+        "graph_ms_per_step": wall_graph / args.steps * 1e3,
+        "p50_substep_ms": float(np.percentile(lat, 50)),
+        "p99_substep_ms": float(np.percentile(lat, 99)),
+        "vertices": N, "elements": E, "colors": sc.n_colors, "tile": list(sc.get_tile()),
+        "gather_frac_of_hbm_peak": achieved / HBM_PEAK_GBS, "gather_achieved_gbs": achieved,
+        "gather_avg_launch_us": avg_launch_s * 1e6,
+    }
+    sc.close()
+    return out
+
+
+def cpu_baseline_leg(steps: int) -> dict:
+    """FP64 oracle VBD (single thread) on a bounded sample of the single-scene workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
+    import octopusengine_amd as oe
     import pyoracle as O
 
-    o = O.Scene(O.TETRA10, xyz, topo, material=O.STABLE_NEOHOOKE, young=5e5, poisson=0.49, iterations=10)
+    xyz, topo = oe.beam_mesh(oe.TETRA10, (64, 16, 16), SIZE)
+    o = O.Scene(O.TETRA10, xyz, topo, material=O.STABLE_NEOHOOKE, young=5e5, poisson=0.49, iterations=ITERS)
     o.fix(np.nonzero(xyz[:, 0] < 1e-6)[0])
     o.box((-10, -5, -10), (10, 5, 10))
     t0 = time.perf_counter()
     for _ in range(steps):
-        o.step(0.005, 1)
+        o.step(DT, 1)
     dt = time.perf_counter() - t0
-    vu = xyz.shape[0] * 10 * steps
     model, ncpu = cpu_info()
-    return {
-        "value": vu / dt,
-        "unit": "vertex-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"{steps} steps (x10 iterations) of the same P2 64x16x16 scene, FP64 oracle, 1 thread; "
-                  f"host {model}, {ncpu} cpus visible",
-        "seconds": dt,
-    }, o.positions()
+    return {"value": xyz.shape[0] * ITERS * steps / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps x {ITERS} iterations of the P2 64x16x16 beam (140481 vertices) with the "
+                      f"FP64 oracle, 1 thread, host '{model}' ({ncpu} cpus visible)",
+            "seconds": dt}
 
 
-def run_cpu_xpbd_reference() -> dict:
-    """Reference CPU XPBD path (float, single thread) on its own config:
-    P1 16x4x4 beam (425 vertices, 1536 tets), NeoHooke E=1e6 nu=0.45,
-    dt=0.01, 20 substeps, 100 steps."""
+def cpu_xpbd_reference_leg() -> dict:
+    """Reference CPU XPBD path (float, single thread) on its own configuration:
+    P1 16x4x4 beam (425 vertices, 1536 tets), NeoHooke E = 1e6, nu = 0.45,
+    dt = 0.01, 20 substeps, 100 steps."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import octopusengine_amd as oe
     import pyoracle as O
 
     xyz, topo = oe.beam_mesh(oe.TETRA, (16, 4, 4), SIZE)
     fixed = np.nonzero(xyz[:, 0] < 1e-6)[0]
-    c = O.CpuXPBD(O.TETRA, xyz, topo, fixed, material=O.NEOHOOKE, young=1e6, poisson=0.45, iterations=1,
-                  substeps=20)
+    c = O.CpuXPBD(O.TETRA, xyz, topo, fixed, material=O.NEOHOOKE, young=1e6, poisson=0.45, iterations=1, substeps=20)
     t0 = time.perf_counter()
     for _ in range(100):
         c.step(0.01)
     dt = time.perf_counter() - t0
     return {"steps_per_s": 100 / dt, "constraint_projections_per_s": 1536 * 2 * 20 * 100 / dt,
-            "vertex_substeps_per_s": 425 * 20 * 100 / dt, "seconds": dt, "cores": 1,
-            "tip_y": float(c.positions()[:, 1].min())}
-
-
-def plain_vbd(args) -> dict:
-    import octopusengine_amd as oe
-
-    eng = oe.Engine(0)
-    xyz, topo = oe.beam_mesh(oe.TETRA10, (64, 16, 16), SIZE)
-    t_setup = time.perf_counter()
-    sc = oe.Scene(eng, oe.TETRA10, xyz, topo, material=oe.STABLE_NEOHOOKE, young=5e5, poisson=0.49, density=1000.0,
-                  distribution=oe.MASS_FEMSHAPE, iterations=10)
-    n_fixed = sc.fix_plane((1e-6, 0.0, 0.0), (-1.0, 0.0, 0.0))
-    sc.box_limit((-10, -5, -10), (10, 5, 10))
-    if args.tile:
-        sc.set_tile(*[int(v) for v in args.tile.split(",")])
-    setup_s = time.perf_counter() - t_setup
-    N, E, S, Q = sc.n_vertices, sc.n_elements, sc.S, sc.Q
-    dt, iters = 0.005, 10
-
-    for _ in range(args.warmup):
-        sc.step(dt, 1)
-    eng.synchronize()
-
-    # timed region: eager launches with HIP events around every gather launch
-    sc.set_timing(True)
-    sc.get_timing()
-    eng.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sc.step(dt, 1)
-    eng.synchronize()
-    wall = time.perf_counter() - t0
-    gather_ms, launches = sc.get_timing()
-    sc.set_timing(False)
-
-    # the same steps replayed from a captured HIP graph (launch overhead removed)
-    sc.set_graph(True)
-    sc.step(dt, 1)
-    eng.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sc.step(dt, 1)
-    eng.synchronize()
-    wall_graph = time.perf_counter() - t0
-
-    # per-substep latency: step + synchronise, host clock
-    lat = []
-    for _ in range(min(args.steps, 200)):
-        t = time.perf_counter()
-        sc.step(dt, 1)
-        eng.synchronize()
-        lat.append((time.perf_counter() - t) * 1e3)
-    sc.set_graph(False)
-    p = sc.positions()
-    finite = bool(np.isfinite(p).all())
-
-    vu_per_step = N * iters
-    value = vu_per_step * args.steps / wall
-    bytes_iter = algorithmic_bytes_per_iteration(N, E, S, Q)
-    bytes_per_launch = bytes_iter * iters * args.steps / max(launches, 1)
-    avg_launch_s = gather_ms / 1e3 / max(launches, 1)
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    out = {
-        "metric": "VBD vertex-updates/s (plain, one scene)",
-        "value": value,
-        "unit": "vertex-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic beam mesh generated in-process (no dataset)",
-        "config": {"workload": "plain VBD, P2 beam 64x16x16 (140481 vertices), SNH E=5e5 nu=0.49, "
-                               "dt=0.005, 10 iterations, 1 scene",
-                   "vertices": N, "elements": E, "colors": sc.n_colors, "fixed": n_fixed,
-                   "tile": list(sc.get_tile()), "launch": "eager + per-colour HIP events"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_vertex_gather (VBD colour pass)",
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_s * 1e6,
-                     "launches": launches},
-        "p50_substep_ms": float(np.percentile(lat, 50)),
-        "p99_substep_ms": float(np.percentile(lat, 99)),
-        "graph_value": vu_per_step * args.steps / wall_graph,
-        "graph_ms_per_step": wall_graph / args.steps * 1e3,
-        "setup_s": setup_s,
-        "positions_finite": finite,
-    }
-    eng_objs = (sc, eng)
-    return out, xyz, topo, eng_objs
+            "vertex_substeps_per_s": 425 * 20 * 100 / dt, "seconds": dt, "cores": 1}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--tile", default="", help="lanes,qpl override of the gather tile")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 4))
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-scene leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; scenes shard across GPUs only by design "
-                                   "(see DESIGN.md), --gpus must be 1"}))
+        print(json.dumps({"error": "one GPU per process: scenes shard across GPUs by scene (design only); "
+                                   "run with --gpus 1"}))
         sys.exit(2)
-    out, xyz, topo, keep = plain_vbd(args)
+    import octopusengine_amd as oe
+
+    eng = oe.Engine(0)
+    bat = batched_leg(args, eng)
+    scenes = bat.pop("_scenes")
+    roof = bat.pop("roofline")
+    out = {
+        "metric": "VBD vertex-updates/s, batched (256 mixed P2/Q2 scenes) vs plain per-scene loop",
+        "value": bat["value"],
+        "unit": "vertex-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": bat["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic beam meshes generated in-process from a fixed seed (no dataset)",
+        "config": {"workload": f"batched VBD: {bat['scenes']} beams ({bat['p2_scenes']} P2 / "
+                               f"{bat['scenes'] - bat['p2_scenes']} Q2), nx~U[8,64], ny,nz~U[4,16], "
+                               f"E~logU[1e5,1e7], nu~U[0.3,0.49], SNH, dt=0.005, 10 iterations, seed {args.seed}",
+                   "vertices": bat["vertices"], "vertex_updates_per_step": bat["vertex_updates_per_step"],
+                   "gather_launches_per_step": bat["gather_launches_per_step"],
+                   "launch": "eager, HIP events around every gather launch"},
+        "roofline": roof,
+        "p50_substep_ms": bat["p50_substep_ms"],
+        "p99_substep_ms": bat["p99_substep_ms"],
+        "batched": {k: bat[k] for k in ("graph_value", "graph_ms_per_step", "plain_value", "plain_ms_per_step",
+                                        "speedup_vs_plain", "setup_s", "positions_finite")},
+    }
+    del scenes
+    if not args.no_single:
+        out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
     if not args.no_cpu:
-        cpu, _ = run_cpu_baseline(xyz, topo)
-        out["cpu_baseline"] = cpu
-        out["cpu_reference_xpbd_config1"] = run_cpu_xpbd_reference()
+        out["cpu_baseline"] = cpu_baseline_leg(args.cpu_steps)
+        out["cpu_reference_xpbd_p1_16x4x4"] = cpu_xpbd_reference_leg()
     print(json.dumps(out), flush=True)
 
 

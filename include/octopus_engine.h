@@ -56,6 +56,8 @@ enum { OE_XPBD_XPBD = 0, OE_XPBD_QUADRATURE = 1, OE_XPBD_OPTISHARED = 2, OE_XPBD
 
 typedef struct oe_engine oe_engine;
 typedef struct oe_scene oe_scene;
+typedef struct oe_batch oe_batch;
+typedef struct oe_server oe_server;
 
 /* Scene description: the arguments of Cuda_{VBD,XPBD}_FEM_Dynamic /
  * Cuda_FEM_Explicit_Dynamic plus the mesh they are attached to. */
@@ -128,6 +130,22 @@ oe_status oe_scene_set_state(oe_scene* s, const float* p, const float* last_p, c
 /* Gather-kernel timing with HIP events around every colour launch. */
 oe_status oe_scene_set_timing(oe_scene* s, int32_t enable);
 oe_status oe_scene_get_timing(oe_scene* s, double* total_ms, int64_t* launches);
+
+/* ------------------------------------------------- batched stepping (many scenes) */
+/* Many VBD scenes stepped together: one launch per global colour index and
+ * element type covers every scene of the batch (the reference steps scene after
+ * scene, src/Core/Engine.cpp:15-21).  Each scene keeps its own results exactly
+ * as if stepped alone.  The scenes must outlive the batch; a batch holds no
+ * copy of their state. */
+oe_status oe_batch_create(oe_engine* e, oe_scene* const* scenes, int32_t n, oe_batch** out);
+oe_status oe_batch_destroy(oe_batch* b);
+oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps);
+oe_status oe_batch_set_graph(oe_batch* b, int32_t enable);
+oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl);
+oe_status oe_batch_set_timing(oe_batch* b, int32_t enable);
+oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches);
+/* Gather launches and vertex-updates per substep; algorithmic gather bytes per substep. */
+oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_updates, double* gather_bytes);
 
 /* ------------------------------------------------- host-side (no GPU needed) */
 /* Beam of nx*ny*nz cells of size sx*sy*sz; pass NULL arrays to query sizes. */

@@ -29,7 +29,7 @@ from ._lib import (HEXA, HEXA27, MASS_FEMSHAPE, MASS_UNIFORM, NEOHOOKE, NODES, S
                    XPBD_QUADRATURE, XPBD_XPBD, OEError)
 
 __all__ = [
-    "Engine", "Scene", "beam_mesh", "vbd_scene", "xpbd_scene", "explicit_scene", "OEError",
+    "Engine", "Scene", "Batch", "make_scene", "beam_mesh", "vbd_scene", "xpbd_scene", "explicit_scene", "OEError",
     "host_shape", "host_color_primal", "host_color_dual", "host_fem_constants", "host_masses",
     "TETRA", "HEXA", "TETRA10", "TETRA20", "HEXA27", "NEOHOOKE", "STABLE_NEOHOOKE",
     "MASS_UNIFORM", "MASS_FEMSHAPE", "SOLVER_VBD", "SOLVER_XPBD", "SOLVER_EXPLICIT",
@@ -268,6 +268,67 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many VBD scenes stepped together (one launch per global colour and
+    element type).  The scenes must stay alive while the batch exists."""
+
+    def __init__(self, engine: Engine, scenes):
+        self.engine = engine
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.handle for s in self.scenes])
+        h = C.c_void_p()
+        L.check(L.lib.oe_batch_create(engine.handle, arr, len(self.scenes), C.byref(h)))
+        self._h = h
+
+    def step(self, dt: float, substeps: int = 1):
+        L.check(L.lib.oe_batch_step(self._h, dt, substeps))
+
+    def set_graph(self, enable: bool):
+        L.check(L.lib.oe_batch_set_graph(self._h, int(enable)))
+
+    def set_tile(self, element: int, lanes: int, qpl: int):
+        L.check(L.lib.oe_batch_set_tile(self._h, element, lanes, qpl))
+
+    def set_timing(self, enable: bool):
+        L.check(L.lib.oe_batch_set_timing(self._h, int(enable)))
+
+    def get_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        L.check(L.lib.oe_batch_get_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def info(self):
+        nl, vu, gb = C.c_int32(), C.c_int64(), C.c_double()
+        L.check(L.lib.oe_batch_info(self._h, C.byref(nl), C.byref(vu), C.byref(gb)))
+        return {"gather_launches": nl.value, "vertex_updates": vu.value, "gather_bytes": gb.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib.oe_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_scene(engine, element, cells, size=(1.0, 0.25, 0.25), *, young=5e5, poisson=0.49,
+               material=STABLE_NEOHOOKE, iterations=10, fix_x0=True, box=True, **kw):
+    """A beam scene as the benchmarks use it: x = 0 face fixed (Plane at x = 1e-6
+    facing -x), box limit (-10,-5,-10)..(10,5,10)."""
+    xyz, topo = beam_mesh(element, cells, size)
+    s = Scene(engine, element, xyz, topo, material=material, young=young, poisson=poisson, iterations=iterations,
+              **kw)
+    s._solver = kw.get("solver", SOLVER_VBD)
+    if fix_x0:
+        s.fix_plane((1e-6, 0.0, 0.0), (-1.0, 0.0, 0.0))
+    if box:
+        s.box_limit((-10.0, -5.0, -10.0), (10.0, 5.0, 10.0))
+    return s
 
 
 def vbd_scene(engine, element, xyz, topo, density, distrib, young, poisson, material, iteration=30,

@@ -14,7 +14,7 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("OE_LIB", Path(__file__).resolve().parent / "liboe.so"))
 
 if not LIB_PATH.exists():
-    raise ImportError(f"{LIB_PATH} not found: build it with `python -m octopusengine_amd.build`")
+    raise ImportError(f"{LIB_PATH} not found: build it with `python octopusengine_amd/build.py`")
 
 lib = C.CDLL(str(LIB_PATH))
 
@@ -98,6 +98,14 @@ _SIGS = {
     "oe_scene_set_state": (C.c_int32, [P, f32p, f32p, f32p]),
     "oe_scene_set_timing": (C.c_int32, [P, C.c_int32]),
     "oe_scene_get_timing": (C.c_int32, [P, f64p, i64p]),
+    "oe_batch_create": (C.c_int32, [P, C.POINTER(P), C.c_int32, C.POINTER(P)]),
+    "oe_batch_destroy": (C.c_int32, [P]),
+    "oe_batch_step": (C.c_int32, [P, C.c_float, C.c_int32]),
+    "oe_batch_set_graph": (C.c_int32, [P, C.c_int32]),
+    "oe_batch_set_tile": (C.c_int32, [P, C.c_int32, C.c_int32, C.c_int32]),
+    "oe_batch_set_timing": (C.c_int32, [P, C.c_int32]),
+    "oe_batch_get_timing": (C.c_int32, [P, f64p, i64p]),
+    "oe_batch_info": (C.c_int32, [P, i32p, i64p, f64p]),
     "oe_host_beam_mesh": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
                                       i32p, i32p, f32p, i32p]),
     "oe_host_shape": (C.c_int32, [C.c_int32, i32p, i32p, f64p, f64p, f64p]),

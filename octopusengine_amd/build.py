@@ -2,7 +2,8 @@
 
 Objects go to octopusengine_amd/_build/, the library to
 octopusengine_amd/liboe.so (in-tree, so it travels with the source tree).
-Run as ``python -m octopusengine_amd.build`` or call :func:`build`.
+Run as ``python octopusengine_amd/build.py`` (a path, not ``-m``: importing the
+package loads the library being rebuilt) or call :func:`build`.
 """
 from __future__ import annotations
 
@@ -31,7 +32,7 @@ HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]
 
 HOST_SOURCES = ["oe_shapes.cpp", "oe_mesh.cpp", "oe_color.cpp", "oe_fem.cpp"]
-HIP_SOURCES = ["oe_gpu.cpp", "oe_scene.cpp", "oe_capi.cpp", "oe_elementwise.hip", "oe_xpbd.hip"]
+HIP_SOURCES = ["oe_gpu.cpp", "oe_scene.cpp", "oe_batch.cpp", "oe_capi.cpp", "oe_elementwise.hip", "oe_xpbd.hip"]
 GATHER_VARIANTS = [(4, 1), (8, 8), (10, 4), (20, 11), (27, 27)]
 EXTRA_HIP_SOURCES: list[str] = []
 
@@ -47,7 +48,8 @@ def _jobs() -> list[tuple[list[str], Path]]:
         jobs.append(([HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)], obj))
     for s, q in GATHER_VARIANTS:
         obj = OUT_DIR / f"oe_gather_{s}_{q}.o"
-        defs = [f"-DOE_GATHER_S={s}", f"-DOE_GATHER_Q={q}", f"-DOE_GATHER_FN=launch_gather_{s}_{q}"]
+        defs = [f"-DOE_GATHER_S={s}", f"-DOE_GATHER_Q={q}", f"-DOE_GATHER_FN=launch_gather_{s}_{q}",
+                f"-DOE_GATHER_BATCH_FN=launch_batch_gather_{s}_{q}"]
         jobs.append(([HIPCC, *HIP_FLAGS, *defs, "-c", str(CSRC / "oe_gather.hip"), "-o", str(obj)], obj))
     return jobs
 

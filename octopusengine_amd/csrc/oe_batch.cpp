@@ -1,0 +1,286 @@
+#include "oe_batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace oe {
+
+namespace {
+constexpr int kBlock = 256;
+GPU_Box_Limit* find_box(const Scene& s) {
+    GPU_Box_Limit* box = nullptr;
+    for (const auto& c : s.integrator->_constraints) {
+        if (auto* b = dynamic_cast<GPU_Box_Limit*>(c.get())) {
+            if (box) throw std::invalid_argument("batched scenes support one box limit");
+            box = b;
+        }
+    }
+    return box;
+}
+}  // namespace
+
+std::map<int, GatherTile> default_tiles() {
+    std::map<int, GatherTile> t;
+    for (Element e : {Tetra, Hexa, Tetra10, Tetra20, Hexa27}) t[e] = default_tile(e);
+    return t;
+}
+
+void check_batchable(const Scene& s) {
+    if (!s.vbd_fem) throw std::invalid_argument("batched stepping needs VBD scenes");
+    if (s.vbd_fem->version == Better_Coloration)
+        throw std::invalid_argument("batched stepping needs a proper colouring (not Better_Coloration)");
+    if (s.cfg.rho > 0.f) throw std::invalid_argument("batched stepping does not run the Chebyshev acceleration");
+    if (s.vbd_fem->shuffle) throw std::invalid_argument("batched stepping uses the ascending colour order");
+    for (const auto& c : s.integrator->_constraints)
+        if (!dynamic_cast<GPU_Fix_Constraint*>(c.get()) && !dynamic_cast<GPU_Box_Limit*>(c.get()))
+            throw std::invalid_argument("unsupported constraint in a batched scene");
+    (void)find_box(s);
+}
+
+std::vector<unsigned char> fixed_flags(const Scene& s) {
+    std::vector<unsigned char> f(s.ps->n, 0);
+    if (s.fix)
+        for (int id : s.fix->ids) f[id] = 1;
+    return f;
+}
+
+SceneDev make_scene_dev(Scene& s, const unsigned char* fixed) {
+    check_batchable(s);
+    SceneDev d{};
+    d.ps = s.ps->get_parameters();
+    d.fem = s.fem->fem_view();
+    d.own = OwnersView{s.vbd_fem->d_slot_vid.get(), s.vbd_fem->d_slot_nbo.get(), s.vbd_fem->d_own.get()};
+    d.mt = s.fem->mt;
+    auto* vbd = dynamic_cast<GPU_VBD*>(s.integrator.get());
+    if (!vbd) throw std::logic_error("VBD scene without a GPU_VBD integrator");
+    d.y = vbd->y.get();
+    d.damping = s.vbd_fem->damping;
+    d.iterations = vbd->iteration;
+    for (int k = 0; k < 3; ++k) d.g[k] = vbd->gravity[k];
+    d.fixed = s.fix ? fixed : nullptr;
+    if (s.fix) {
+        std::memcpy(d.fix_com, s.fix->t.com, sizeof(d.fix_com));
+        std::memcpy(d.fix_offset, s.fix->t.offset, sizeof(d.fix_offset));
+        std::memcpy(d.fix_R, s.fix->t.R, sizeof(d.fix_R));
+    }
+    if (GPU_Box_Limit* box = find_box(s)) {
+        d.has_box = 1;
+        for (int k = 0; k < 3; ++k) {
+            d.box_min[k] = box->pmin[k];
+            d.box_max[k] = box->pmax[k];
+        }
+    }
+    return d;
+}
+
+void Schedule::clear() {
+    segs.clear();
+    block_seg.clear();
+    glaunches.clear();
+    vsegs.clear();
+    vblock_seg.clear();
+    max_iterations = 0;
+    vertex_updates_per_substep = 0;
+    algorithmic_bytes_per_substep = 0.0;
+}
+
+void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& table_index,
+                     const std::map<int, GatherTile>& tiles) {
+    clear();
+    int max_colors = 0;
+    for (Scene* s : scenes) {
+        max_colors = std::max(max_colors, s->vbd_fem->nb_colors());
+        max_iterations = std::max(max_iterations, s->cfg.iterations);
+        vertex_updates_per_substep += static_cast<int64_t>(s->ps->n) * s->cfg.iterations;
+        const int S = s->fem->S, Q = s->fem->Q;
+        algorithmic_bytes_per_substep +=
+            (static_cast<double>(s->fem->E) * S * (40.0 * Q + 16.0 * S + 8.0) + 56.0 * s->ps->n) * s->cfg.iterations;
+    }
+    static const Element order[] = {Tetra, Tetra10, Tetra20, Hexa, Hexa27};
+    for (int c = 0; c < max_colors; ++c) {
+        for (Element el : order) {
+            GLaunch gl{};
+            gl.color = c;
+            gl.block_base = static_cast<int>(block_seg.size());
+            bool any = false;
+            for (size_t i = 0; i < scenes.size(); ++i) {
+                Scene* s = scenes[i];
+                if (s->mesh.element != el || c >= s->vbd_fem->nb_colors()) continue;
+                const auto it = tiles.find(el);
+                const GatherTile t = it != tiles.end() ? it->second : default_tile(el);
+                const int first = s->vbd_fem->owners.color_begin[c];
+                const int count = s->vbd_fem->owners.color_begin[c + 1] - first;
+                if (count <= 0) continue;
+                if (!any) {
+                    gl.S = s->fem->S;
+                    gl.Q = s->fem->Q;
+                    gl.L = t.L;
+                    gl.QL = t.QL;
+                    any = true;
+                }
+                const int nb = static_cast<int>((static_cast<int64_t>(count) * gl.L + kBlock - 1) / kBlock);
+                const int sid = static_cast<int>(segs.size());
+                segs.push_back(BatchSeg{table_index[i], first, count, static_cast<int>(block_seg.size())});
+                block_seg.insert(block_seg.end(), nb, sid);
+                gl.slots += count;
+                for (int k = first; k < first + count; ++k) gl.incidences += s->vbd_fem->owners.slot_nb[k];
+            }
+            gl.nblocks = static_cast<int>(block_seg.size()) - gl.block_base;
+            if (any) glaunches.push_back(gl);
+        }
+    }
+    for (size_t i = 0; i < scenes.size(); ++i) {
+        const int n = scenes[i]->ps->n;
+        const int nb = (n + kBlock - 1) / kBlock;
+        const int sid = static_cast<int>(vsegs.size());
+        vsegs.push_back(VSeg{table_index[i], n, static_cast<int>(vblock_seg.size())});
+        vblock_seg.insert(vblock_seg.end(), nb, sid);
+    }
+}
+
+namespace {
+template <typename T>
+void upload_grow(Engine* eng, DBuf<T>& buf, const std::vector<T>& h, hipStream_t s) {
+    if (buf.size() < h.size() || !buf.get()) buf.allocate(&eng->pool, std::max<size_t>(h.size() + h.size() / 2, 16));
+    if (!h.empty()) OE_HIP(hipMemcpyAsync(buf.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+}  // namespace
+
+void DeviceSchedule::upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync) {
+    upload_grow(eng, segs, s.segs, stream);
+    upload_grow(eng, block_seg, s.block_seg, stream);
+    upload_grow(eng, vsegs, s.vsegs, stream);
+    upload_grow(eng, vblock_seg, s.vblock_seg, stream);
+    if (sync) OE_HIP(hipStreamSynchronize(stream));
+}
+
+void GatherTimer::before(hipStream_t s) {
+    if (!enabled) return;
+    while (pool.size() < used + 2) {
+        hipEvent_t ev;
+        OE_HIP(hipEventCreate(&ev));
+        pool.push_back(ev);
+    }
+    OE_HIP(hipEventRecord(pool[used], s));
+}
+
+void GatherTimer::after(hipStream_t s) {
+    if (!enabled) return;
+    OE_HIP(hipEventRecord(pool[used + 1], s));
+    used += 2;
+}
+
+void GatherTimer::collect(hipStream_t s) {
+    if (used == 0) return;
+    OE_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i + 1 < used; i += 2) {
+        float t = 0.f;
+        OE_HIP(hipEventElapsedTime(&t, pool[i], pool[i + 1]));
+        ms += t;
+        launches++;
+    }
+    used = 0;
+}
+
+GatherTimer::~GatherTimer() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const SceneDev* d_scenes, float dt, int substeps,
+                     hipStream_t stream, GatherTimer* timer) {
+    if (sch.vsegs.empty()) return;
+    const float h = dt / static_cast<float>(substeps);
+    BatchVArgs v{d_scenes, dev.vsegs.get(), dev.vblock_seg.get(), h, 0};
+    const int vblocks = static_cast<int>(sch.vblock_seg.size());
+    BatchGatherArgs g{d_scenes, dev.segs.get(), dev.block_seg.get(), 0, 0, h};
+    for (int sub = 0; sub < substeps; ++sub) {
+        launch_batch_predict(v, vblocks, stream);
+        for (int it = 0; it < sch.max_iterations; ++it) {
+            g.iteration = it;
+            for (const Schedule::GLaunch& gl : sch.glaunches) {
+                g.block_base = gl.block_base;
+                if (timer) timer->before(stream);
+                if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, g, gl.nblocks, stream))
+                    throw std::invalid_argument("unsupported gather tile in batch");
+                if (timer) timer->after(stream);
+            }
+            v.iteration = it;
+            launch_batch_fix_box(v, vblocks, stream);
+        }
+        launch_batch_velocity(v, vblocks, stream);
+    }
+}
+
+Batch::Batch(Engine* e, std::vector<Scene*> sc) : eng(e), scenes(std::move(sc)), tiles(default_tiles()) {
+    if (scenes.empty()) throw std::invalid_argument("empty batch");
+    for (Scene* s : scenes) {
+        if (!s) throw std::invalid_argument("null scene in batch");
+        if (s->eng != eng) throw std::invalid_argument("scene belongs to another engine");
+        check_batchable(*s);
+    }
+    std::vector<SceneDev> host;
+    flags.reserve(scenes.size());
+    for (Scene* s : scenes) {
+        flags.emplace_back(&eng->pool, fixed_flags(*s));
+        host.push_back(make_scene_dev(*s, flags.back().get()));
+    }
+    d_scenes = DBuf<SceneDev>(&eng->pool, host);
+    rebuild();
+}
+
+Batch::~Batch() { drop_graph(); }
+
+void Batch::rebuild() {
+    drop_graph();
+    std::vector<int> idx(scenes.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = static_cast<int>(i);
+    sched.build(scenes, idx, tiles);
+    dsched.upload(eng, sched, eng->stream, true);
+}
+
+void Batch::set_tile(Element e, GatherTile t) {
+    tiles[e] = t;
+    rebuild();
+}
+
+void Batch::set_graph(bool on) {
+    if (!on) drop_graph();
+    use_graph = on;
+}
+
+void Batch::drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+void Batch::step(float dt, int substeps) {
+    if (!(dt > 0.f) || substeps <= 0) throw std::invalid_argument("step needs dt > 0 and substeps > 0");
+    if (!use_graph || timer.enabled) {
+        launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, timer.enabled ? &timer : nullptr);
+    } else {
+        if (!graph_exec || graph_dt != dt || graph_sub != substeps) {
+            drop_graph();
+            OE_HIP(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, nullptr);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(eng->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            OE_HIP(hipStreamEndCapture(eng->stream, &graph));
+            OE_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+            graph_dt = dt;
+            graph_sub = substeps;
+        }
+        OE_HIP(hipGraphLaunch(graph_exec, eng->stream));
+    }
+    OE_LAUNCH_CHECK();
+    for (Scene* s : scenes) s->steps_done++;
+}
+
+}  // namespace oe

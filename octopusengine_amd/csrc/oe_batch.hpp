@@ -1,0 +1,110 @@
+// Batched VBD stepping of many independent scenes on one GPU.
+//
+// The reference steps each scene (entity) on its own, one launch per colour
+// (src/GPU/VBD/GPU_VBD_FEM.cu:712-756, driven scene after scene by
+// src/Core/Engine.cpp:15-21).  Here a launch covers colour c of every scene
+// that has a colour c: scenes are independent, so colour c of scene A and
+// colour c of scene B can be solved concurrently, and a launch carries enough
+// vertices to fill the chip.  Per step the launch sequence is
+//   predictor (all vertices of all scenes)
+//   for each iteration:  for each global colour c, for each element type:
+//                          gather + local solve of colour c
+//                        fixation + box limit (all vertices, one fused kernel)
+//   velocity update
+// Each scene still sees the reference's order: ascending colours, constraints
+// after every sweep.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "oe_gather.hpp"
+#include "oe_scene.hpp"
+
+namespace oe {
+
+// Launch plan for a set of scenes.  Scene indices refer to a SceneDev table.
+struct Schedule {
+    struct GLaunch {
+        int color, S, Q, L, QL;
+        int block_base, nblocks;
+        int64_t slots;   // vertex slots covered (all scenes)
+        int64_t incidences;
+    };
+    std::vector<BatchSeg> segs;
+    std::vector<int> block_seg;
+    std::vector<GLaunch> glaunches;   // ascending colour, then element type
+    std::vector<VSeg> vsegs;
+    std::vector<int> vblock_seg;
+    int max_iterations = 0;
+    int64_t vertex_updates_per_substep = 0;
+    double algorithmic_bytes_per_substep = 0.0;   // gather kernels, all iterations
+
+    void clear();
+    // scenes[i] is placed at SceneDev table index table_index[i]
+    void build(const std::vector<Scene*>& scenes, const std::vector<int>& table_index,
+               const std::map<int, GatherTile>& tiles);
+};
+
+// Device copies of a schedule's tables (grown on demand, reused across uploads).
+struct DeviceSchedule {
+    DBuf<BatchSeg> segs;
+    DBuf<int> block_seg;
+    DBuf<VSeg> vsegs;
+    DBuf<int> vblock_seg;
+    void upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync);
+};
+
+// Validates that a scene can be stepped in a batch and describes it for the
+// batched kernels.  `fixed` is the scene's per-vertex fixation flag array.
+SceneDev make_scene_dev(Scene& s, const unsigned char* fixed);
+std::vector<unsigned char> fixed_flags(const Scene& s);
+void check_batchable(const Scene& s);
+
+struct GatherTimer {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    double ms = 0.0;
+    int64_t launches = 0;
+    void before(hipStream_t s);
+    void after(hipStream_t s);
+    void collect(hipStream_t s);
+    ~GatherTimer();
+};
+
+// Enqueue `substeps` substeps of dt/substeps for every scene of the schedule.
+void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const SceneDev* d_scenes, float dt, int substeps,
+                     hipStream_t stream, GatherTimer* timer);
+
+std::map<int, GatherTile> default_tiles();
+
+struct Batch {
+    Batch(Engine* eng, std::vector<Scene*> scenes);
+    ~Batch();
+    void step(float dt, int substeps);
+    void set_graph(bool on);
+    void set_tile(Element e, GatherTile t);
+    void rebuild();
+
+    Engine* eng;
+    std::vector<Scene*> scenes;
+    std::vector<DBuf<unsigned char>> flags;
+    DBuf<SceneDev> d_scenes;
+    std::map<int, GatherTile> tiles;
+    Schedule sched;
+    DeviceSchedule dsched;
+    GatherTimer timer;
+
+private:
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    float graph_dt = 0.f;
+    int graph_sub = 0;
+    void drop_graph();
+};
+
+}  // namespace oe

@@ -7,6 +7,7 @@
 #include <new>
 #include <string>
 
+#include "oe_batch.hpp"
 #include "oe_scene.hpp"
 
 struct oe_engine {
@@ -15,6 +16,9 @@ struct oe_engine {
 struct oe_scene {
     std::unique_ptr<oe::Scene> scene;
     oe_engine* owner;
+};
+struct oe_batch {
+    std::unique_ptr<oe::Batch> batch;
 };
 
 namespace {
@@ -325,6 +329,86 @@ oe_status oe_scene_get_timing(oe_scene* s, double* total_ms, int64_t* launches) 
         if (launches) *launches = f->gather_launches;
         f->gather_ms = 0.0;
         f->gather_launches = 0;
+    });
+}
+
+// ------------------------------------------------------------------ batches
+oe_status oe_batch_create(oe_engine* e, oe_scene* const* scenes, int32_t n, oe_batch** out) {
+    return guard([&] {
+        OE_REQUIRE(e && scenes && n > 0 && out, "bad argument");
+        std::vector<oe::Scene*> v;
+        for (int32_t i = 0; i < n; ++i) {
+            OE_REQUIRE(scenes[i], "null scene");
+            OE_REQUIRE(scenes[i]->owner == e, "scene belongs to another engine");
+            v.push_back(scenes[i]->scene.get());
+        }
+        auto* b = new oe_batch();
+        try {
+            b->batch = std::make_unique<oe::Batch>(e->eng.get(), std::move(v));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+oe_status oe_batch_destroy(oe_batch* b) {
+    return guard([&] { delete b; });
+}
+
+oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        b->batch->step(dt, substeps);
+    });
+}
+
+oe_status oe_batch_set_graph(oe_batch* b, int32_t enable) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        b->batch->set_graph(enable != 0);
+    });
+}
+
+oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl) {
+    return guard([&] {
+        OE_REQUIRE(b && oe::element_supported(element), "bad argument");
+        OE_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "lanes must be 4..64, power of 2");
+        const int Q = oe::elem_nb_quadrature(static_cast<oe::Element>(element));
+        OE_REQUIRE(qpl >= 1 && Q % qpl == 0, "quadrature points per lane must divide Q");
+        OE_REQUIRE(qpl == 1 || qpl == Q || (Q == 4 && qpl == 2) || (Q == 27 && (qpl == 3 || qpl == 9)),
+                   "tile not instantiated");
+        b->batch->set_tile(static_cast<oe::Element>(element), oe::GatherTile{lanes, qpl});
+    });
+}
+
+oe_status oe_batch_set_timing(oe_batch* b, int32_t enable) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        b->batch->timer.enabled = enable != 0;
+    });
+}
+
+oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        oe::GatherTimer& t = b->batch->timer;
+        t.collect(b->batch->eng->stream);
+        if (gather_ms) *gather_ms = t.ms;
+        if (launches) *launches = t.launches;
+        t.ms = 0.0;
+        t.launches = 0;
+    });
+}
+
+oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_updates, double* gather_bytes) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        const oe::Schedule& s = b->batch->sched;
+        if (gather_launches) *gather_launches = static_cast<int32_t>(s.glaunches.size()) * s.max_iterations;
+        if (vertex_updates) *vertex_updates = s.vertex_updates_per_substep;
+        if (gather_bytes) *gather_bytes = s.algorithmic_bytes_per_substep;
     });
 }
 

@@ -7,10 +7,9 @@
 // identical traversal order without a node allocation per entry.
 //
 // DSAT (Graph.h:321-371) keeps the uncoloured vertices in a std::set ordered by
-// (saturation, remaining degree, id), largest first.  Here the same order is
-// served by a max-heap with lazy invalidation: a vertex's remaining degree
-// strictly decreases with every update, so (deg, id) identifies the current
-// entry and stale entries are skipped when popped.
+// (saturation, remaining degree, id), largest first, re-keying a neighbour on
+// every visit.  Here the same order is served by an indexed max-heap whose
+// entries are re-sifted in place, and saturation sets are bitsets.
 #include "oe_host.hpp"
 
 #include <algorithm>
@@ -89,23 +88,85 @@ MultiGraph build_dual_vertex_graph(Element e, const std::vector<int32_t>& topo, 
     return finish_csr(E, cnt, std::move(adj));
 }
 
+namespace {
+
+// Max-heap of vertices keyed by (saturation, remaining degree, id) with
+// in-place key updates (a vertex's key changes every time a neighbour is
+// coloured; the heap position is tracked so the entry is re-sifted instead of
+// re-inserted).
+class DsatHeap {
+public:
+    DsatHeap(int n, const std::vector<int32_t>& sat, const std::vector<int32_t>& deg)
+        : sat_(sat), deg_(deg), heap_(n), pos_(n) {
+        for (int v = 0; v < n; ++v) heap_[v] = pos_[v] = v;
+        for (int i = n / 2 - 1; i >= 0; --i) down(i);
+    }
+    bool empty() const { return heap_.empty(); }
+    int pop() {
+        const int top = heap_[0];
+        swap_at(0, static_cast<int>(heap_.size()) - 1);
+        heap_.pop_back();
+        pos_[top] = -1;
+        if (!heap_.empty()) down(0);
+        return top;
+    }
+    // key of v only grows in saturation or shrinks in degree: sift both ways
+    void update(int v) {
+        const int i = pos_[v];
+        if (i < 0) return;
+        up(i);
+        down(pos_[v]);
+    }
+
+private:
+    bool above(int a, int b) const {
+        if (sat_[a] != sat_[b]) return sat_[a] > sat_[b];
+        if (deg_[a] != deg_[b]) return deg_[a] > deg_[b];
+        return a > b;
+    }
+    void swap_at(int i, int j) {
+        std::swap(heap_[i], heap_[j]);
+        pos_[heap_[i]] = i;
+        pos_[heap_[j]] = j;
+    }
+    void up(int i) {
+        while (i > 0) {
+            const int p = (i - 1) / 2;
+            if (!above(heap_[i], heap_[p])) break;
+            swap_at(i, p);
+            i = p;
+        }
+    }
+    void down(int i) {
+        const int n = static_cast<int>(heap_.size());
+        while (true) {
+            const int l = 2 * i + 1, r = l + 1;
+            int b = i;
+            if (l < n && above(heap_[l], heap_[b])) b = l;
+            if (r < n && above(heap_[r], heap_[b])) b = r;
+            if (b == i) return;
+            swap_at(i, b);
+            i = b;
+        }
+    }
+    const std::vector<int32_t>& sat_;
+    const std::vector<int32_t>& deg_;
+    std::vector<int> heap_, pos_;
+};
+
+}  // namespace
+
 Coloration color_dsat(const MultiGraph& g) {
     const int n = g.n;
-    std::vector<int32_t> colors(n, -1);
-    std::vector<int32_t> deg(n);
-    std::vector<std::vector<int32_t>> sat(n);   // sorted distinct neighbour colours
-    using Key = std::tuple<int32_t, int32_t, int32_t>;  // (sat, deg, vertex), max first
-    std::priority_queue<Key> heap;
-    for (int v = 0; v < n; ++v) {
-        deg[v] = g.degree(v);
-        heap.emplace(0, deg[v], v);
-    }
+    constexpr int W = 4;   // saturation bitset: up to 256 colours
+    std::vector<int32_t> colors(n, -1), deg(n), sat(n, 0);
+    std::vector<uint64_t> bits(static_cast<size_t>(n) * W, 0);
+    for (int v = 0; v < n; ++v) deg[v] = g.degree(v);
+    DsatHeap heap(n, sat, deg);
     std::vector<char> used(n + 1, 0);
     int max_color = -1;
     while (!heap.empty()) {
-        const auto [s, d, v] = heap.top();
-        heap.pop();
-        if (colors[v] != -1 || s != static_cast<int32_t>(sat[v].size()) || d != deg[v]) continue;
+        const int v = heap.pop();
         for (int64_t i = g.off[v]; i < g.off[v + 1]; ++i) {
             const int32_t c = colors[g.adj[i]];
             if (c != -1) used[c] = 1;
@@ -116,16 +177,20 @@ Coloration color_dsat(const MultiGraph& g) {
             const int32_t cc = colors[g.adj[i]];
             if (cc != -1) used[cc] = 0;
         }
+        if (c >= 64 * W) throw std::runtime_error("DSAT: more than 256 colours");
         colors[v] = c;
         max_color = std::max(max_color, c);
         for (int64_t i = g.off[v]; i < g.off[v + 1]; ++i) {
             const int32_t u = g.adj[i];
             if (colors[u] != -1) continue;
-            auto& su = sat[u];
-            auto it = std::lower_bound(su.begin(), su.end(), c);
-            if (it == su.end() || *it != c) su.insert(it, c);
+            uint64_t& word = bits[static_cast<size_t>(u) * W + (c >> 6)];
+            const uint64_t bit = 1ull << (c & 63);
+            if (!(word & bit)) {
+                word |= bit;
+                sat[u]++;
+            }
             deg[u]--;
-            heap.emplace(static_cast<int32_t>(su.size()), deg[u], u);
+            heap.update(u);
         }
     }
     Coloration out;

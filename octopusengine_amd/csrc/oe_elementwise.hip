@@ -10,6 +10,7 @@
 //   box limit                 src/GPU/GPU_Constraint.cu:70-81
 //   Jacobi colour copy        src/GPU/VBD/GPU_VBD_FEM.cu:402-413
 #include "oe_elementwise.hpp"
+#include "oe_gather.hpp"
 
 namespace oe {
 
@@ -139,6 +140,90 @@ __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* sl
     ps.p[vid] = p_new[vid];
 }
 
+
+// ------------------------------------------------------------ batched forms
+// One launch covers every vertex of every scene of a batch; each block belongs
+// to one scene (VSeg), whose description is read with scalar loads.
+__device__ __forceinline__ const SceneDev& vseg_scene(const BatchVArgs& b, int& i, int& ok) {
+    const VSeg seg = b.segs[b.block_seg[blockIdx.x]];
+    i = (blockIdx.x - seg.block_begin) * kBlock + threadIdx.x;
+    ok = i < seg.count;
+    return b.scenes[seg.scene];
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_predict(BatchVArgs b) {
+    int i, ok;
+    const SceneDev& sd = vseg_scene(b, i, ok);
+    if (!ok) return;
+    const PSView ps = sd.ps;
+    const float dt = b.dt;
+    const float4 p = ps.p[i];
+    const float4 lp = ps.last_p[i];
+    const float4 f = ps.f[i];
+    const float w = ps.w[i];
+    const float inv_dt = 1.f / dt;
+    const float ax = sd.g[0] + f.x * w, ay = sd.g[1] + f.y * w, az = sd.g[2] + f.z * w;
+    const float4 yv = f4(p.x + ((p.x - lp.x) * inv_dt + ax * dt) * dt, p.y + ((p.y - lp.y) * inv_dt + ay * dt) * dt,
+                         p.z + ((p.z - lp.z) * inv_dt + az * dt) * dt, p.w);
+    sd.y[i] = yv;
+    ps.last_p[i] = p;
+    ps.p[i] = yv;
+    ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+}
+
+// fixation then box limit, fused: a fixed vertex is snapped and masked, every
+// other vertex (mask 1) is clamped - the same result as the two kernels in
+// sequence, since the box skips masked vertices.
+__global__ __launch_bounds__(kBlock) void k_batch_fix_box(BatchVArgs b) {
+    int i, ok;
+    const SceneDev& sd = vseg_scene(b, i, ok);
+    if (!ok || b.iteration >= sd.iterations) return;
+    const PSView ps = sd.ps;
+    if (sd.fixed && sd.fixed[i]) {
+        const float4 x0 = ps.init_p[i];
+        const float dx = x0.x - sd.fix_com[0], dy = x0.y - sd.fix_com[1], dz = x0.z - sd.fix_com[2];
+        const float* R = sd.fix_R;
+        const float rx = R[0] * dx + R[3] * dy + R[6] * dz;
+        const float ry = R[1] * dx + R[4] * dy + R[7] * dz;
+        const float rz = R[2] * dx + R[5] * dy + R[8] * dz;
+        const float4 p = ps.p[i];
+        ps.p[i] = f4(sd.fix_offset[0] + sd.fix_com[0] + rx, sd.fix_offset[1] + sd.fix_com[1] + ry,
+                     sd.fix_offset[2] + sd.fix_com[2] + rz, p.w);
+        ps.v[i] = f4(0.f, 0.f, 0.f, 0.f);
+        ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+        ps.mask[i] = 0;
+        return;
+    }
+    if (!sd.has_box || ps.mask[i] == 0) return;
+    float4 p = ps.p[i];
+    p.x = fminf(fmaxf(p.x, sd.box_min[0]), sd.box_max[0]);
+    p.y = fminf(fmaxf(p.y, sd.box_min[1]), sd.box_max[1]);
+    p.z = fminf(fmaxf(p.z, sd.box_min[2]), sd.box_max[2]);
+    ps.p[i] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_velocity(BatchVArgs b) {
+    int i, ok;
+    const SceneDev& sd = vseg_scene(b, i, ok);
+    if (!ok) return;
+    const PSView ps = sd.ps;
+    const float4 p = ps.p[i], lp = ps.last_p[i];
+    ps.v[i] = f4((p.x - lp.x) / b.dt, (p.y - lp.y) / b.dt, (p.z - lp.z) / b.dt, 0.f);
+}
+
+}  // namespace
+
+void launch_batch_predict(const BatchVArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_batch_predict, dim3(nblocks), dim3(kBlock), 0, s, b);
+}
+void launch_batch_fix_box(const BatchVArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_batch_fix_box, dim3(nblocks), dim3(kBlock), 0, s, b);
+}
+void launch_batch_velocity(const BatchVArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_batch_velocity, dim3(nblocks), dim3(kBlock), 0, s, b);
+}
+
+namespace {
 }  // namespace
 
 void launch_vbd_predict(const PSView& ps, float4* y, float4* prev_it_p, float dt, const float g[3], hipStream_t s) {

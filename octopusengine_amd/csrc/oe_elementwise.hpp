@@ -24,4 +24,22 @@ void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipS
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,
                         hipStream_t s);
 
+struct SceneDev;
+// All vertices of one scene, covered by blocks [block_begin, ...) of a launch.
+struct VSeg {
+    int scene;
+    int count;
+    int block_begin;
+};
+struct BatchVArgs {
+    const SceneDev* scenes;
+    const VSeg* segs;
+    const int* block_seg;
+    float dt;
+    int iteration;
+};
+void launch_batch_predict(const BatchVArgs& b, int nblocks, hipStream_t s);
+void launch_batch_fix_box(const BatchVArgs& b, int nblocks, hipStream_t s);
+void launch_batch_velocity(const BatchVArgs& b, int nblocks, hipStream_t s);
+
 }  // namespace oe

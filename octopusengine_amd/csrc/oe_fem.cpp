@@ -61,6 +61,30 @@ FemConstants build_fem_constants(const HostMesh& m) {
     return fc;
 }
 
+std::vector<double> element_volumes(const HostMesh& m) {
+    const ShapeTable& st = shape_table(m.element);
+    const int S = st.S, Q = st.Q, E = m.nb_elements();
+    std::vector<double> vol(E, 0.0);
+    for (int e = 0; e < E; ++e) {
+        const int32_t* t = &m.topo[static_cast<size_t>(e) * S];
+        for (int q = 0; q < Q; ++q) {
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int k = 0; k < S; ++k) {
+                const double* d = &st.dN[(q * S + k) * 3];
+                const float* X = &m.xyz[3 * static_cast<size_t>(t[k])];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) J[i][j] += static_cast<double>(X[i]) * d[j];
+            }
+            const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                               J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            if (det == 0.0) throw std::invalid_argument("degenerate element (det J = 0)");
+            vol[e] += std::fabs(det) * st.weights[q];
+        }
+    }
+    return vol;
+}
+
 std::vector<float> compute_fem_mass(const HostMesh& m, const FemConstants& fc, float density,
                                     Mass_Distribution distrib) {
     const int N = m.nb_vertices();

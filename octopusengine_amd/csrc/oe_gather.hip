@@ -21,12 +21,10 @@
 
 namespace oe {
 
+// Gather + local solve of one vertex slot by a group of L lanes.
 template <int S, int Q, int L, int QL>
-__global__ __launch_bounds__(256) void k_vertex_gather(GatherArgs a) {
+__device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int lane) {
     constexpr int QC = Q / QL;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int grp = gt / L;
-    const int lane = gt & (L - 1);
     if (grp >= a.count) return;                 // whole group leaves together
     const int slot = a.first_slot + grp;
     const int vid = a.own.slot_vid[slot];
@@ -83,6 +81,37 @@ __global__ __launch_bounds__(256) void k_vertex_gather(GatherArgs a) {
         a.ps.p[vid] = out;
 }
 
+template <int S, int Q, int L, int QL>
+__global__ __launch_bounds__(256) void k_vertex_gather(GatherArgs a) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    gather_body<S, Q, L, QL>(a, gt / L, gt & (L - 1));
+}
+
+// Batched form: one launch covers one colour of many scenes.  Every block
+// belongs to one (scene, colour) segment, so the scene's description is read
+// once per block with scalar loads.
+template <int S, int Q, int L, int QL>
+__global__ __launch_bounds__(256) void k_batch_gather(BatchGatherArgs b) {
+    const int blk = b.block_base + blockIdx.x;
+    const BatchSeg seg = b.segs[b.block_seg[blk]];
+    const SceneDev& sd = b.scenes[seg.scene];
+    if (b.iteration >= sd.iterations) return;
+    GatherArgs a;
+    a.ps = sd.ps;
+    a.fem = sd.fem;
+    a.own = sd.own;
+    a.mt = sd.mt;
+    a.y = sd.y;
+    a.p_new = nullptr;
+    a.damping = sd.damping;
+    a.dt = b.dt;
+    a.first_slot = seg.first;
+    a.count = seg.count;
+    a.mode = GATHER_VBD_INPLACE;
+    const int local = (blk - seg.block_begin) * 256 + threadIdx.x;
+    gather_body<S, Q, L, QL>(a, local / L, local & (L - 1));
+}
+
 namespace {
 
 template <int S, int Q, int L, int QL>
@@ -92,6 +121,24 @@ void launch_one(const GatherArgs& a, hipStream_t s) {
     const int block = 256;
     const int grid = static_cast<int>((threads + block - 1) / block);
     hipLaunchKernelGGL((k_vertex_gather<S, Q, L, QL>), dim3(grid), dim3(block), 0, s, a);
+}
+
+template <int S, int Q, int L, int QL>
+void launch_batch_one(const BatchGatherArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL((k_batch_gather<S, Q, L, QL>), dim3(nblocks), dim3(256), 0, s, b);
+}
+
+template <int S, int Q, int QL>
+bool launch_batch_L(int L, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
+    switch (L) {
+        case 4: launch_batch_one<S, Q, 4, QL>(b, nblocks, s); return true;
+        case 8: launch_batch_one<S, Q, 8, QL>(b, nblocks, s); return true;
+        case 16: launch_batch_one<S, Q, 16, QL>(b, nblocks, s); return true;
+        case 32: launch_batch_one<S, Q, 32, QL>(b, nblocks, s); return true;
+        case 64: launch_batch_one<S, Q, 64, QL>(b, nblocks, s); return true;
+        default: return false;
+    }
 }
 
 template <int S, int Q, int QL>
@@ -121,6 +168,20 @@ bool OE_GATHER_FN(int L, int QL, const GatherArgs& a, hipStream_t s) {
 #if OE_GATHER_Q == 27
     if (QL == 3) return launch_L<S, Q, 3>(L, a, s);
     if (QL == 9) return launch_L<S, Q, 9>(L, a, s);
+#endif
+    return false;
+}
+
+bool OE_GATHER_BATCH_FN(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
+    constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
+    if (QL == 1) return launch_batch_L<S, Q, 1>(L, b, nblocks, s);
+    if (QL == Q) return launch_batch_L<S, Q, Q>(L, b, nblocks, s);
+#if OE_GATHER_Q == 4
+    if (QL == 2) return launch_batch_L<S, Q, 2>(L, b, nblocks, s);
+#endif
+#if OE_GATHER_Q == 27
+    if (QL == 3) return launch_batch_L<S, Q, 3>(L, b, nblocks, s);
+    if (QL == 9) return launch_batch_L<S, Q, 9>(L, b, nblocks, s);
 #endif
     return false;
 }

@@ -27,11 +27,65 @@ struct GatherArgs {
     int mode;
 };
 
+// Everything a batched kernel needs to know about one scene; an array of these
+// lives in device memory and blocks index it by scene.
+struct SceneDev {
+    PSView ps;
+    FemView fem;
+    OwnersView own;
+    MaterialView mt;
+    float4* y;
+    float damping;
+    int iterations;
+    float g[3];
+    // constraints: fixed vertices (per-vertex flag, may be null) then box limit
+    const unsigned char* fixed;
+    float fix_com[3];
+    float fix_offset[3];
+    float fix_R[9];
+    int has_box;
+    float box_min[3];
+    float box_max[3];
+};
+
+// A run of `count` consecutive slots (first = first slot) of one scene, covered
+// by blocks [block_begin, block_begin + nblocks) of a launch.
+struct BatchSeg {
+    int scene;
+    int first;
+    int count;
+    int block_begin;
+};
+
+struct BatchGatherArgs {
+    const SceneDev* scenes;
+    const BatchSeg* segs;
+    const int* block_seg;   // block -> segment id
+    int block_base;         // first entry of block_seg used by this launch
+    int iteration;          // scenes with fewer iterations skip the launch
+    float dt;
+};
+
 bool launch_gather_4_1(int L, int QL, const GatherArgs& a, hipStream_t s);
 bool launch_gather_8_8(int L, int QL, const GatherArgs& a, hipStream_t s);
 bool launch_gather_10_4(int L, int QL, const GatherArgs& a, hipStream_t s);
 bool launch_gather_20_11(int L, int QL, const GatherArgs& a, hipStream_t s);
 bool launch_gather_27_27(int L, int QL, const GatherArgs& a, hipStream_t s);
+
+bool launch_batch_gather_4_1(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
+bool launch_batch_gather_8_8(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
+bool launch_batch_gather_10_4(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
+bool launch_batch_gather_20_11(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
+bool launch_batch_gather_27_27(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
+
+inline bool launch_batch_gather(int S, int Q, int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
+    if (S == 4 && Q == 1) return launch_batch_gather_4_1(L, QL, b, nblocks, s);
+    if (S == 8 && Q == 8) return launch_batch_gather_8_8(L, QL, b, nblocks, s);
+    if (S == 10 && Q == 4) return launch_batch_gather_10_4(L, QL, b, nblocks, s);
+    if (S == 20 && Q == 11) return launch_batch_gather_20_11(L, QL, b, nblocks, s);
+    if (S == 27 && Q == 27) return launch_batch_gather_27_27(L, QL, b, nblocks, s);
+    return false;
+}
 
 // Dispatch on (S, Q); returns false for an unsupported tile shape.
 inline bool launch_gather(int S, int Q, int L, int QL, const GatherArgs& a, hipStream_t s) {

@@ -207,8 +207,8 @@ GatherTile default_tile(Element e) {
     switch (e) {
         case Tetra: return {8, 1};
         case Hexa: return {16, 1};
-        case Tetra10: return {8, 4};
-        case Tetra20: return {32, 1};
+        case Tetra10: return {16, 2};
+        case Tetra20: return {64, 1};
         case Hexa27: return {64, 1};
         default: return {16, 1};
     }

@@ -105,6 +105,7 @@ struct FemConstants {
     std::vector<double> elem_volume;  // E
 };
 FemConstants build_fem_constants(const HostMesh& m);
+std::vector<double> element_volumes(const HostMesh& m);
 
 // Reference src/Dynamic/FEM/FEM_Shape.cpp:86-121.
 std::vector<float> compute_fem_mass(const HostMesh& m, const FemConstants& fc, float density,

@@ -14,7 +14,11 @@ Scene::Scene(Engine* e, HostMesh m, const SceneConfig& c) : eng(e), mesh(std::mo
     if (!(cfg.young > 0.f) || !(cfg.poisson > -1.f && cfg.poisson < 0.5f) || !(cfg.density > 0.f))
         throw std::invalid_argument("invalid material parameters");
     {
-        const FemConstants fc = build_fem_constants(mesh);
+        // masses need only the element volumes: sum over q of |det J| w
+        FemConstants fc;
+        fc.S = S;
+        fc.E = mesh.nb_elements();
+        fc.elem_volume = element_volumes(mesh);
         masses = compute_fem_mass(mesh, fc, cfg.density, cfg.distribution);
         for (float mv : masses)
             if (!(mv > 0.f)) throw std::invalid_argument("vertex with zero mass (not used by any element)");

@@ -20,7 +20,9 @@ namespace oe {
 
 namespace {
 
-// energy e and dE/dF for constraint m of the XPBD energy split
+// energy e and dE/dF for constraint m of the XPBD energy split.  The
+// NeoHooke split uses alpha = 1 + mu/lambda (PBD_ContinuousMaterial.h:61-65,
+// GPU_PBD_FEM_Materials.cu:105-107), unlike the GPU VBD NeoHooke stress.
 __device__ __forceinline__ void xpbd_material(int material, int m, float alpha, const float F[9], float P[9],
                                               float& e) {
     float C[9];
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256) void k_xpbd_v0(XpbdArgs a) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
             mat_mul(Jx, JXi, F);
-            xpbd_material(a.mt.material, m, a.mt.alpha, F, P, e);
+            xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
             scale_stress(P, JXi, rec[9], Ps);
             C += e * rec[9];
 #pragma unroll
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
             mat_mul(Jx, JXi, F);
-            xpbd_material(a.mt.material, m, a.mt.alpha, F, P, e);
+            xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
             scale_stress(P, JXi, rec[9], Ps);
             e_loc = e * rec[9];
 #pragma unroll

@@ -1,0 +1,91 @@
+"""XPBD and explicit integrators on the GPU vs the FP64 oracle."""
+import numpy as np
+import pytest
+
+from conftest import beam_diag
+from scenes import make_pair, max_err
+
+pytestmark = pytest.mark.gpu
+
+DIAG = beam_diag()
+TOL1 = 1e-5 * DIAG
+TOL100 = 1e-3 * DIAG
+
+XPBD_CASES = [
+    (3, (16, 4, 4), 0),   # P1 -> sqrt form (V0), mask-aware
+    (7, (4, 2, 2), 2),    # P2 OptiShared -> energy form (V2)
+    (9, (4, 2, 2), 2),    # Q2 OptiShared
+    (8, (3, 1, 1), 2),    # P3 OptiShared
+    (7, (4, 2, 2), 0),    # P2 with the XPBD version -> V0 for every element
+    (6, (6, 2, 2), 1),    # Q1 Quadrature -> V2 arithmetic
+]
+
+
+@pytest.mark.parametrize("case", XPBD_CASES, ids=[f"el{c[0]}-v{c[2]}" for c in XPBD_CASES])
+def test_xpbd_one_frame(engine, case):
+    el, cells, version = case
+    g, o, rest, _ = make_pair(engine, el, cells, solver=1, material=2, young=1e6, poisson=0.45,
+                              xpbd_version=version)
+    # one frame: dt = 0.01 with 50 substeps (config 3 XPBD)
+    g.step(0.01, 50)
+    o.step(0.01, 50)
+    err = max_err(g.positions(), o.positions())
+    disp = float(np.abs(o.positions() - rest).max())
+    print(f"XPBD el={el} v={version} 1 frame err={err:.3e} disp={disp:.3e}", flush=True)
+    assert disp > 1e-4
+    assert err <= TOL1
+
+
+def test_xpbd_many_frames(engine):
+    g, o, rest, _ = make_pair(engine, 3, (16, 4, 4), solver=1, material=2, young=1e6, poisson=0.45,
+                              xpbd_version=0)
+    for _ in range(30):
+        g.step(0.01, 20)
+        o.step(0.01, 20)
+    err = max_err(g.positions(), o.positions())
+    print(f"XPBD P1 30 frames err={err:.3e}", flush=True)
+    assert err <= TOL100
+
+
+def test_xpbd_stable_neohooke_and_damping(engine):
+    g, o, _, _ = make_pair(engine, 7, (4, 2, 2), solver=1, material=3, young=1e6, poisson=0.45,
+                           global_damping=2.0, xpbd_version=2)
+    for _ in range(3):
+        g.step(0.01, 10)
+        o.step(0.01, 10)
+    assert max_err(g.positions(), o.positions()) <= TOL1 * 3
+
+
+def test_xpbd_optigroup_high_order_is_inert(engine):
+    """OptiGroup launches no projection for non-P1 elements (reference
+    GPU_PBD_FEM.cu:671-692): only gravity and the constraints act."""
+    g, o, _, _ = make_pair(engine, 7, (4, 2, 2), solver=1, material=2, young=1e6, poisson=0.45,
+                           xpbd_version=3)
+    g.step(0.01, 5)
+    o.step(0.01, 5)
+    assert max_err(g.positions(), o.positions()) <= TOL1
+
+
+EXPLICIT_CASES = [(9, (4, 2, 2)), (8, (3, 1, 1)), (7, (4, 2, 2)), (3, (8, 2, 2))]
+
+
+@pytest.mark.parametrize("case", EXPLICIT_CASES, ids=[f"el{c[0]}" for c in EXPLICIT_CASES])
+def test_explicit_substeps(engine, case):
+    el, cells = case
+    # config 3: NeoHooke E=1e6 nu=0.45, +-1e-3 m noise, dt = 1e-5
+    g, o, rest, _ = make_pair(engine, el, cells, solver=2, material=2, young=1e6, poisson=0.45, noise=1e-3, seed=7)
+    g.step(2e-3, 200)      # 200 substeps of 1e-5
+    o.step(2e-3, 200)
+    err = max_err(g.positions(), o.positions())
+    moved = max_err(o.positions(), rest)
+    print(f"explicit el={el} 200 substeps err={err:.3e} moved={moved:.3e}", flush=True)
+    assert moved > 1e-4
+    assert err <= TOL1
+
+
+def test_explicit_damping(engine):
+    g, o, _, _ = make_pair(engine, 9, (4, 2, 2), solver=2, material=3, young=1e6, poisson=0.45, damping=1e-4,
+                           noise=1e-3, seed=3)
+    g.step(1e-3, 100)
+    o.step(1e-3, 100)
+    assert max_err(g.positions(), o.positions()) <= TOL1
