@@ -147,6 +147,31 @@ oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches)
 /* Gather launches and vertex-updates per substep; algorithmic gather bytes per substep. */
 oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_updates, double* gather_bytes);
 
+/* ------------------------------------------------- serving (continuous batching) */
+/* A request loop over HBM-resident VBD scenes (no counterpart in the
+ * reference, which steps every scene once per frame).  A request advances one
+ * scene by step(dt, substeps).  A worker thread batches the oldest pending
+ * requests whose scenes are idle (one request per scene per batch, at most
+ * max_batch scenes) into one batched step on one of `streams` streams, so
+ * requests that arrive while others are on the GPU are admitted as soon as a
+ * stream and their scene are free and batches overlap on the device.  Scenes
+ * are admitted against the engine's memory budget when they are created
+ * (oe_scene_create returns OE_ERR_BUDGET beyond it). */
+oe_status oe_server_create(oe_engine* e, int32_t max_batch, int32_t streams, oe_server** out);
+oe_status oe_server_destroy(oe_server* sv);
+/* Register a scene; returns its slot id.  The scene must outlive the server. */
+oe_status oe_server_add_scene(oe_server* sv, oe_scene* s, int32_t* slot);
+/* Thread-safe; records the arrival time of the request. */
+oe_status oe_server_submit(oe_server* sv, int32_t slot, float dt, int32_t substeps, uint64_t* ticket);
+/* Wait until every submitted request is complete. */
+oe_status oe_server_drain(oe_server* sv);
+/* Since creation: completed requests, latency percentiles (ms, arrival to
+ * completion), vertex-updates, batches and the mean batch size. */
+oe_status oe_server_stats(oe_server* sv, int64_t* completed, double* p50_ms, double* p99_ms, double* max_ms,
+                          int64_t* vertex_updates, int64_t* batches, double* mean_batch);
+/* Per-request latencies in completion order (ms), up to cap entries. */
+oe_status oe_server_latencies(oe_server* sv, double* out, int64_t cap, int64_t* n);
+
 /* ------------------------------------------------- host-side (no GPU needed) */
 /* Beam of nx*ny*nz cells of size sx*sy*sz; pass NULL arrays to query sizes. */
 oe_status oe_host_beam_mesh(int32_t element, int32_t nx, int32_t ny, int32_t nz, float sx, float sy, float sz,

@@ -29,7 +29,7 @@ from ._lib import (HEXA, HEXA27, MASS_FEMSHAPE, MASS_UNIFORM, NEOHOOKE, NODES, S
                    XPBD_QUADRATURE, XPBD_XPBD, OEError)
 
 __all__ = [
-    "Engine", "Scene", "Batch", "make_scene", "beam_mesh", "vbd_scene", "xpbd_scene", "explicit_scene", "OEError",
+    "Engine", "Scene", "Batch", "Server", "make_scene", "beam_mesh", "vbd_scene", "xpbd_scene", "explicit_scene", "OEError",
     "host_shape", "host_color_primal", "host_color_dual", "host_fem_constants", "host_masses",
     "TETRA", "HEXA", "TETRA10", "TETRA20", "HEXA27", "NEOHOOKE", "STABLE_NEOHOOKE",
     "MASS_UNIFORM", "MASS_FEMSHAPE", "SOLVER_VBD", "SOLVER_XPBD", "SOLVER_EXPLICIT",
@@ -307,6 +307,57 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             L.lib.oe_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over resident VBD scenes."""
+
+    def __init__(self, engine: Engine, max_batch: int = 64, streams: int = 2):
+        self.engine = engine
+        h = C.c_void_p()
+        L.check(L.lib.oe_server_create(engine.handle, max_batch, streams, C.byref(h)))
+        self._h = h
+        self._scenes = []
+
+    def add_scene(self, scene: Scene) -> int:
+        slot = C.c_int32()
+        L.check(L.lib.oe_server_add_scene(self._h, scene.handle, C.byref(slot)))
+        self._scenes.append(scene)
+        return slot.value
+
+    def submit(self, slot: int, dt: float, substeps: int = 1) -> int:
+        t = C.c_uint64()
+        L.check(L.lib.oe_server_submit(self._h, slot, dt, substeps, C.byref(t)))
+        return t.value
+
+    def drain(self):
+        L.check(L.lib.oe_server_drain(self._h))
+
+    def stats(self):
+        c, vu, nb = C.c_int64(), C.c_int64(), C.c_int64()
+        p50, p99, mx, mb = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        L.check(L.lib.oe_server_stats(self._h, C.byref(c), C.byref(p50), C.byref(p99), C.byref(mx), C.byref(vu),
+                                      C.byref(nb), C.byref(mb)))
+        return {"completed": c.value, "p50_ms": p50.value, "p99_ms": p99.value, "max_ms": mx.value,
+                "vertex_updates": vu.value, "batches": nb.value, "mean_batch": mb.value}
+
+    def latencies(self):
+        n = C.c_int64()
+        L.check(L.lib.oe_server_latencies(self._h, None, 0, C.byref(n)))
+        out = np.empty(n.value, np.float64)
+        L.check(L.lib.oe_server_latencies(self._h, L.dptr(out), n.value, C.byref(n)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib.oe_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -106,6 +106,13 @@ _SIGS = {
     "oe_batch_set_timing": (C.c_int32, [P, C.c_int32]),
     "oe_batch_get_timing": (C.c_int32, [P, f64p, i64p]),
     "oe_batch_info": (C.c_int32, [P, i32p, i64p, f64p]),
+    "oe_server_create": (C.c_int32, [P, C.c_int32, C.c_int32, C.POINTER(P)]),
+    "oe_server_destroy": (C.c_int32, [P]),
+    "oe_server_add_scene": (C.c_int32, [P, P, i32p]),
+    "oe_server_submit": (C.c_int32, [P, C.c_int32, C.c_float, C.c_int32, u64p]),
+    "oe_server_drain": (C.c_int32, [P]),
+    "oe_server_stats": (C.c_int32, [P, i64p, f64p, f64p, f64p, i64p, i64p, f64p]),
+    "oe_server_latencies": (C.c_int32, [P, f64p, C.c_int64, i64p]),
     "oe_host_beam_mesh": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
                                       i32p, i32p, f32p, i32p]),
     "oe_host_shape": (C.c_int32, [C.c_int32, i32p, i32p, f64p, f64p, f64p]),

@@ -124,7 +124,7 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                 segs.push_back(BatchSeg{table_index[i], first, count, static_cast<int>(block_seg.size())});
                 block_seg.insert(block_seg.end(), nb, sid);
                 gl.slots += count;
-                for (int k = first; k < first + count; ++k) gl.incidences += s->vbd_fem->owners.slot_nb[k];
+                gl.incidences += s->vbd_fem->owners.color_incidences[c];
             }
             gl.nblocks = static_cast<int>(block_seg.size()) - gl.block_base;
             if (any) glaunches.push_back(gl);

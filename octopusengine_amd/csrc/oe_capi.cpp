@@ -2,12 +2,14 @@
 // an oe_status; the message is kept per thread for oe_last_error().
 #include "../../include/octopus_engine.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <string>
 
 #include "oe_batch.hpp"
+#include "oe_server.hpp"
 #include "oe_scene.hpp"
 
 struct oe_engine {
@@ -19,6 +21,10 @@ struct oe_scene {
 };
 struct oe_batch {
     std::unique_ptr<oe::Batch> batch;
+};
+struct oe_server {
+    std::unique_ptr<oe::Server> server;
+    oe_engine* owner;
 };
 
 namespace {
@@ -409,6 +415,75 @@ oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_u
         if (gather_launches) *gather_launches = static_cast<int32_t>(s.glaunches.size()) * s.max_iterations;
         if (vertex_updates) *vertex_updates = s.vertex_updates_per_substep;
         if (gather_bytes) *gather_bytes = s.algorithmic_bytes_per_substep;
+    });
+}
+
+// ------------------------------------------------------------------ server
+oe_status oe_server_create(oe_engine* e, int32_t max_batch, int32_t streams, oe_server** out) {
+    return guard([&] {
+        OE_REQUIRE(e && out, "bad argument");
+        auto* s = new oe_server();
+        try {
+            s->server = std::make_unique<oe::Server>(e->eng.get(), max_batch, streams);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        s->owner = e;
+        *out = s;
+    });
+}
+
+oe_status oe_server_destroy(oe_server* sv) {
+    return guard([&] { delete sv; });
+}
+
+oe_status oe_server_add_scene(oe_server* sv, oe_scene* s, int32_t* slot) {
+    return guard([&] {
+        OE_REQUIRE(sv && s, "bad argument");
+        OE_REQUIRE(s->owner == sv->owner, "scene belongs to another engine");
+        const int id = sv->server->add_scene(s->scene.get());
+        if (slot) *slot = id;
+    });
+}
+
+oe_status oe_server_submit(oe_server* sv, int32_t slot, float dt, int32_t substeps, uint64_t* ticket) {
+    return guard([&] {
+        OE_REQUIRE(sv, "null server");
+        const uint64_t t = sv->server->submit(slot, dt, substeps);
+        if (ticket) *ticket = t;
+    });
+}
+
+oe_status oe_server_drain(oe_server* sv) {
+    return guard([&] {
+        OE_REQUIRE(sv, "null server");
+        sv->server->drain();
+    });
+}
+
+oe_status oe_server_stats(oe_server* sv, int64_t* completed, double* p50, double* p99, double* mx, int64_t* vu,
+                          int64_t* batches, double* mean_batch) {
+    return guard([&] {
+        OE_REQUIRE(sv, "null server");
+        const oe::Server::Stats s = sv->server->stats();
+        if (completed) *completed = s.completed;
+        if (p50) *p50 = s.p50_ms;
+        if (p99) *p99 = s.p99_ms;
+        if (mx) *mx = s.max_ms;
+        if (vu) *vu = s.vertex_updates;
+        if (batches) *batches = s.batches;
+        if (mean_batch) *mean_batch = s.mean_batch;
+    });
+}
+
+oe_status oe_server_latencies(oe_server* sv, double* out, int64_t cap, int64_t* n) {
+    return guard([&] {
+        OE_REQUIRE(sv && cap >= 0, "bad argument");
+        const std::vector<double> v = sv->server->latencies();
+        const int64_t k = std::min<int64_t>(cap, static_cast<int64_t>(v.size()));
+        if (out && k > 0) std::memcpy(out, v.data(), static_cast<size_t>(k) * sizeof(double));
+        if (n) *n = static_cast<int64_t>(v.size());
     });
 }
 

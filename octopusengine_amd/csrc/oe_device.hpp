@@ -126,12 +126,16 @@ private:
 
 // Particle state, structure of arrays.  Each field is its own array; a
 // position is one 16-byte record so that a gather of a node's position is one
-// vector load.
+// vector load.  p, last_p (and the VBD inertia target y) hold displacements
+// u = x - X from the rest position X = init_p: in FP32 a displacement of a
+// millimetre keeps ~1e-10 m resolution where an absolute coordinate of a metre
+// keeps ~6e-8 m, and deformation gradients are built from differences of
+// these values.
 struct PSView {
     int n;
-    float4* p;
-    float4* last_p;
-    float4* init_p;
+    float4* p;        // displacement
+    float4* last_p;   // displacement at the start of the step
+    float4* init_p;   // rest (absolute) position
     float4* v;
     float4* f;
     const float* m;

@@ -1,6 +1,8 @@
 // Per-vertex kernels of the integrators and constraints.  Each is one pass
 // over SoA particle state with coalesced 16-byte accesses; they are launch- or
-// HBM-bound and carry no reuse.
+// HBM-bound and carry no reuse.  Positions are stored as displacements from
+// the rest position (init_p); the integrators are linear in the positions and
+// read them unchanged, the constraints convert to absolute positions.
 //   VBD predictor             src/GPU/VBD/GPU_VBD.cu:6-18
 //   VBD velocity              src/GPU/VBD/GPU_VBD.cu:20-25
 //   Chebyshev acceleration    src/GPU/VBD/GPU_VBD.cu:27-35
@@ -101,34 +103,46 @@ __global__ __launch_bounds__(kBlock) void k_pbd_velocity(PSView ps, float dt, fl
     ps.v[i] = f4(vx, vy, vz, 0.f);
 }
 
-// target = offset + com + R (init_p - com);  p = target, v = f = 0, mask = 0
+// target = offset + com + R (X - com), X = init_p (rest position);  p = target,
+// v = f = 0, mask = 0.  p holds the displacement x - X, so the stored value is
+// offset + (R d - d) with d = X - com (exactly offset when R = I).
+__device__ __forceinline__ float4 fix_displacement(float4 X, const float com[3], const float off[3], const float R[9],
+                                                   float w) {
+    const float dx = X.x - com[0], dy = X.y - com[1], dz = X.z - com[2];
+    // R is column-major; R d = col0 dx + col1 dy + col2 dz
+    const float rx = R[0] * dx + R[3] * dy + R[6] * dz;
+    const float ry = R[1] * dx + R[4] * dy + R[7] * dz;
+    const float rz = R[2] * dx + R[5] * dy + R[8] * dz;
+    return f4(off[0] + (rx - dx), off[1] + (ry - dy), off[2] + (rz - dz), w);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fix(PSView ps, const int* ids, int n, FixTransform t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int id = ids[i];
-    const float4 x0 = ps.init_p[id];
-    const float dx = x0.x - t.com[0], dy = x0.y - t.com[1], dz = x0.z - t.com[2];
-    // R is column-major; R d = col0 dx + col1 dy + col2 dz
-    const float rx = t.R[0] * dx + t.R[3] * dy + t.R[6] * dz;
-    const float ry = t.R[1] * dx + t.R[4] * dy + t.R[7] * dz;
-    const float rz = t.R[2] * dx + t.R[5] * dy + t.R[8] * dz;
-    const float bx = t.offset[0] + t.com[0], by = t.offset[1] + t.com[1], bz = t.offset[2] + t.com[2];
-    const float4 p = ps.p[id];
-    ps.p[id] = f4(bx + rx, by + ry, bz + rz, p.w);
+    ps.p[id] = fix_displacement(ps.init_p[id], t.com, t.offset, t.R, ps.p[id].w);
     ps.v[id] = f4(0.f, 0.f, 0.f, 0.f);
     ps.f[id] = f4(0.f, 0.f, 0.f, 0.f);
     ps.mask[id] = 0;
+}
+
+// clamp the absolute position X + u into [pmin, pmax]; u is rewritten only
+// when a component is clamped
+__device__ __forceinline__ void box_clamp(float4& u, float4 X, const float pmin[3], const float pmax[3]) {
+    const float x = X.x + u.x, y = X.y + u.y, z = X.z + u.z;
+    if (x < pmin[0] || x > pmax[0]) u.x = fminf(fmaxf(x, pmin[0]), pmax[0]) - X.x;
+    if (y < pmin[1] || y > pmax[1]) u.y = fminf(fmaxf(y, pmin[1]), pmax[1]) - X.y;
+    if (z < pmin[2] || z > pmax[2]) u.z = fminf(fmaxf(z, pmin[2]), pmax[2]) - X.z;
 }
 
 __global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 pmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ps.n) return;
     if (ps.mask[i] == 0) return;
-    float4 p = ps.p[i];
-    p.x = fminf(fmaxf(p.x, pmin.x), pmax.x);
-    p.y = fminf(fmaxf(p.y, pmin.y), pmax.y);
-    p.z = fminf(fmaxf(p.z, pmin.z), pmax.z);
-    ps.p[i] = p;
+    float4 u = ps.p[i];
+    const float lo[3] = {pmin.x, pmin.y, pmin.z}, hi[3] = {pmax.x, pmax.y, pmax.z};
+    box_clamp(u, ps.init_p[i], lo, hi);
+    ps.p[i] = u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* slot_vid, int first, int count,
@@ -180,26 +194,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_fix_box(BatchVArgs b) {
     if (!ok || b.iteration >= sd.iterations) return;
     const PSView ps = sd.ps;
     if (sd.fixed && sd.fixed[i]) {
-        const float4 x0 = ps.init_p[i];
-        const float dx = x0.x - sd.fix_com[0], dy = x0.y - sd.fix_com[1], dz = x0.z - sd.fix_com[2];
-        const float* R = sd.fix_R;
-        const float rx = R[0] * dx + R[3] * dy + R[6] * dz;
-        const float ry = R[1] * dx + R[4] * dy + R[7] * dz;
-        const float rz = R[2] * dx + R[5] * dy + R[8] * dz;
-        const float4 p = ps.p[i];
-        ps.p[i] = f4(sd.fix_offset[0] + sd.fix_com[0] + rx, sd.fix_offset[1] + sd.fix_com[1] + ry,
-                     sd.fix_offset[2] + sd.fix_com[2] + rz, p.w);
+        ps.p[i] = fix_displacement(ps.init_p[i], sd.fix_com, sd.fix_offset, sd.fix_R, ps.p[i].w);
         ps.v[i] = f4(0.f, 0.f, 0.f, 0.f);
         ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
         ps.mask[i] = 0;
         return;
     }
     if (!sd.has_box || ps.mask[i] == 0) return;
-    float4 p = ps.p[i];
-    p.x = fminf(fmaxf(p.x, sd.box_min[0]), sd.box_max[0]);
-    p.y = fminf(fmaxf(p.y, sd.box_min[1]), sd.box_max[1]);
-    p.z = fminf(fmaxf(p.z, sd.box_min[2]), sd.box_max[2]);
-    ps.p[i] = p;
+    float4 u = ps.p[i];
+    box_clamp(u, ps.init_p[i], sd.box_min, sd.box_max);
+    ps.p[i] = u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_velocity(BatchVArgs b) {

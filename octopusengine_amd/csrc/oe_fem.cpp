@@ -132,6 +132,7 @@ OwnersCSR build_owners(const HostMesh& m, const Coloration& col) {
     const int nc = col.nb_color;
     o.color_begin.assign(nc + 1, 0);
     o.color_max_nb.assign(nc, 0);
+    o.color_incidences.assign(nc, 0);
     for (int v = 0; v < N; ++v) {
         const int c = col.colors[v];
         if (c < 0 || c >= nc) throw std::invalid_argument("colour out of range");
@@ -151,6 +152,7 @@ OwnersCSR build_owners(const HostMesh& m, const Coloration& col) {
         for (int64_t i = cnt[v]; i < cnt[v + 1]; ++i) o.own.push_back(inc[i]);
         const int c = col.colors[v];
         o.color_max_nb[c] = std::max(o.color_max_nb[c], o.slot_nb[s]);
+        o.color_incidences[c] += o.slot_nb[s];
     }
     return o;
 }

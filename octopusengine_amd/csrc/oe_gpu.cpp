@@ -55,13 +55,15 @@ void from_f4(const std::vector<float4>& v, std::vector<float>& xyz) {
 GPU_ParticleSystem::GPU_ParticleSystem(Engine* e, const std::vector<float>& xyz, const std::vector<float>& masses)
     : eng(e), n(static_cast<int>(xyz.size() / 3)) {
     if (static_cast<int>(masses.size()) != n) throw std::invalid_argument("mass count != vertex count");
-    const std::vector<float4> P = to_f4(xyz);
+    const std::vector<float4> X = to_f4(xyz);
+    const std::vector<float4> zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
     std::vector<float> inv(n);
     for (int i = 0; i < n; ++i) inv[i] = 1.f / masses[i];   // w = 1/m (GPU_ParticleSystem.cu:11-12)
     MemoryPool* pool = &eng->pool;
-    p = DBuf<float4>(pool, P);
-    last_p = DBuf<float4>(pool, P);
-    init_p = DBuf<float4>(pool, P);
+    rest = xyz;
+    p = DBuf<float4>(pool, zero);        // displacement from the rest position
+    last_p = DBuf<float4>(pool, zero);
+    init_p = DBuf<float4>(pool, X);
     v = DBuf<float4>(pool, std::vector<float4>(n, make_float4(0.f, 0.f, 0.f, 0.f)));
     f = DBuf<float4>(pool, std::vector<float4>(n, make_float4(0.f, 0.f, 0.f, 0.f)));
     m = DBuf<float>(pool, masses);
@@ -77,11 +79,13 @@ void GPU_ParticleSystem::get_position(std::vector<float>& xyz) const {
     std::vector<float4> h;
     p.download(h, eng->stream);
     from_f4(h, xyz);
+    for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = rest[i] + xyz[i];
 }
 void GPU_ParticleSystem::get_prev_position(std::vector<float>& xyz) const {
     std::vector<float4> h;
     last_p.download(h, eng->stream);
     from_f4(h, xyz);
+    for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = rest[i] + xyz[i];
 }
 void GPU_ParticleSystem::get_init_position(std::vector<float>& xyz) const {
     std::vector<float4> h;
@@ -101,16 +105,19 @@ void GPU_ParticleSystem::get_forces(std::vector<float>& xyz) const {
 void GPU_ParticleSystem::get_masks(std::vector<int>& out) const { mask.download(out, eng->stream); }
 void GPU_ParticleSystem::get_mass(std::vector<float>& out) const { m.download(out, eng->stream); }
 void GPU_ParticleSystem::get_inv_mass(std::vector<float>& out) const { w.download(out, eng->stream); }
-void GPU_ParticleSystem::set_position(const std::vector<float>& xyz) { p.upload(to_f4(xyz), eng->stream); }
+std::vector<float> GPU_ParticleSystem::to_displacement(const float* xyz) const {
+    std::vector<float> u(3 * static_cast<size_t>(n));
+    for (size_t i = 0; i < u.size(); ++i) u[i] = static_cast<float>(static_cast<double>(xyz[i]) - rest[i]);
+    return u;
+}
+void GPU_ParticleSystem::set_position(const std::vector<float>& xyz) {
+    p.upload(to_f4(to_displacement(xyz.data())), eng->stream);
+}
 void GPU_ParticleSystem::set_velocity(const std::vector<float>& xyz) { v.upload(to_f4(xyz), eng->stream); }
 void GPU_ParticleSystem::set_state(const float* pp, const float* lp, const float* vv) {
-    auto conv = [&](const float* a) {
-        std::vector<float> t(a, a + 3 * static_cast<size_t>(n));
-        return to_f4(t);
-    };
-    if (pp) p.upload(conv(pp), eng->stream);
-    if (lp) last_p.upload(conv(lp), eng->stream);
-    if (vv) v.upload(conv(vv), eng->stream);
+    if (pp) p.upload(to_f4(to_displacement(pp)), eng->stream);
+    if (lp) last_p.upload(to_f4(to_displacement(lp)), eng->stream);
+    if (vv) v.upload(to_f4(std::vector<float>(vv, vv + 3 * static_cast<size_t>(n))), eng->stream);
 }
 size_t GPU_ParticleSystem::device_bytes() const {
     return p.bytes() + last_p.bytes() + init_p.bytes() + v.bytes() + f.bytes() + m.bytes() + w.bytes() + mask.bytes();

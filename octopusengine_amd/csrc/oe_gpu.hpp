@@ -55,9 +55,11 @@ struct GPU_ParticleSystem {
     void set_velocity(const std::vector<float>& xyz);
     void set_state(const float* p, const float* last_p, const float* v);
     size_t device_bytes() const;
+    std::vector<float> to_displacement(const float* xyz) const;
 
     Engine* eng;
     int n;
+    std::vector<float> rest;   // rest positions X (host copy, 3n)
     DBuf<float4> p, last_p, init_p, v, f;
     DBuf<float> m, w;
     DBuf<int> mask;

@@ -125,6 +125,7 @@ struct OwnersCSR {
     std::vector<int32_t> own;           // E * S packed (eid << 5 | ref)
     std::vector<int32_t> color_begin;   // nb_color + 1 (slot ranges)
     std::vector<int32_t> color_max_nb;  // max incident elements per colour
+    std::vector<int64_t> color_incidences;  // sum of incident elements per colour
 };
 OwnersCSR build_owners(const HostMesh& m, const Coloration& col);
 

@@ -64,6 +64,17 @@ __device__ __forceinline__ void mat_mul(const float A[9], const float B[9], floa
             C[c * 3 + r] = A[0 * 3 + r] * B[c * 3 + 0] + A[1 * 3 + r] * B[c * 3 + 1] + A[2 * 3 + r] * B[c * 3 + 2];
 }
 
+// Deformation gradient from nodal displacements u = x - X:
+//   F = (sum_k x_k (x) dN_k) JX_inv = I + (sum_k u_k (x) dN_k) JX_inv,
+// since sum_k X_k (x) dN_k = J_X.  Working on displacements keeps F accurate
+// to the displacement's precision instead of the absolute position's.
+__device__ __forceinline__ void deformation_gradient(const float Ju[9], const float JXi[9], float F[9]) {
+    mat_mul(Ju, JXi, F);
+    F[0] += 1.f;
+    F[4] += 1.f;
+    F[8] += 1.f;
+}
+
 // cofactor matrix: column c = cross of the two other columns (cyclic)
 __device__ __forceinline__ void cofactor(const float F[9], float C[9]) {
     const float* a = F;       // column 0
@@ -92,7 +103,7 @@ __device__ __forceinline__ float det_from_cof(const float F[9], const float C[9]
     return J;
 }
 
-// One (element, quadrature point) contribution to the force on node r and to
+// One (element, quadrature point) contribution (Jx built from displacements) to the force on node r and to
 // its 3x3 Hessian block, for the Neo-Hookean family used by the reference's
 // GPU VBD / explicit paths (src/GPU/GPU_FEM_Material.cu:25-36,167-207,250-261):
 //   P = mu F + lambda (J - alpha) cof F
@@ -108,7 +119,7 @@ __device__ __forceinline__ void nh_contrib(const float Jx[9], const float* __res
     for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
     const float V = rec[9];
     float F[9];
-    mat_mul(Jx, JXi, F);
+    deformation_gradient(Jx, JXi, F);
     // g_c = sum_l JX_inv(l, c) dN_r[l]
     float g[3];
 #pragma unroll

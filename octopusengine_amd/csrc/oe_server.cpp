@@ -1,0 +1,219 @@
+#include "oe_server.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace oe {
+
+Server::Server(Engine* eng, int max_batch, int nstreams) : eng_(eng), max_batch_(max_batch), tiles_(default_tiles()) {
+    if (max_batch < 1 || nstreams < 1 || nstreams > 8) throw std::invalid_argument("max_batch >= 1, 1 <= streams <= 8");
+    slots_.resize(nstreams);
+    for (StreamSlot& s : slots_) {
+        OE_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        OE_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    }
+    thread_ = std::thread([this] { worker(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (thread_.joinable()) thread_.join();
+    for (StreamSlot& s : slots_) {
+        if (s.stream) {
+            (void)hipStreamSynchronize(s.stream);
+            (void)hipStreamDestroy(s.stream);
+        }
+        if (s.done) (void)hipEventDestroy(s.done);
+    }
+}
+
+int Server::add_scene(Scene* s) {
+    if (!s || s->eng != eng_) throw std::invalid_argument("scene belongs to another engine");
+    check_batchable(*s);
+    std::unique_lock<std::mutex> lk(mu_);
+    if (!error_.empty()) throw std::runtime_error(error_);
+    // the device table is reallocated only while nothing is in flight
+    idle_cv_.wait(lk, [&] { return inflight_.empty() || !error_.empty(); });
+    flags_.emplace_back(&eng_->pool, fixed_flags(*s));
+    host_table_.push_back(make_scene_dev(*s, flags_.back().get()));
+    scenes_.push_back(s);
+    scene_busy_.push_back(0);
+    grow_table();
+    return static_cast<int>(scenes_.size()) - 1;
+}
+
+void Server::grow_table() {
+    if (table_.size() < host_table_.size()) {
+        DBuf<SceneDev> bigger(&eng_->pool, std::max<size_t>(16, host_table_.size() * 2));
+        table_ = std::move(bigger);
+        table_uploaded_ = 0;
+    }
+    const size_t n = host_table_.size() - table_uploaded_;
+    OE_HIP(hipMemcpy(table_.get() + table_uploaded_, host_table_.data() + table_uploaded_, n * sizeof(SceneDev),
+                     hipMemcpyHostToDevice));
+    table_uploaded_ = host_table_.size();
+}
+
+uint64_t Server::submit(int slot, float dt, int substeps) {
+    if (!(dt > 0.f) || substeps <= 0) throw std::invalid_argument("step needs dt > 0 and substeps > 0");
+    uint64_t t;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!error_.empty()) throw std::runtime_error(error_);
+        if (slot < 0 || slot >= static_cast<int>(scenes_.size())) throw std::invalid_argument("unknown scene slot");
+        t = next_ticket_++;
+        pending_.push_back(Request{t, slot, dt, substeps, Clock::now()});
+        submitted_++;
+    }
+    cv_.notify_one();
+    return t;
+}
+
+void Server::drain() {
+    std::unique_lock<std::mutex> lk(mu_);
+    idle_cv_.wait(lk, [&] { return (pending_.empty() && inflight_.empty()) || !error_.empty(); });
+    if (!error_.empty()) throw std::runtime_error(error_);
+}
+
+void Server::retire(bool) {
+    std::vector<Tick> done;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (size_t i = 0; i < inflight_.size();) {
+            const hipError_t q = hipEventQuery(inflight_[i].done);
+            if (q == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            if (q != hipSuccess) throw HipError(q, std::string("tick failed: ") + hipGetErrorString(q));
+            done.push_back(std::move(inflight_[i]));
+            inflight_.erase(inflight_.begin() + static_cast<long>(i));
+        }
+        if (done.empty()) return;
+        const Clock::time_point now = Clock::now();
+        for (Tick& t : done) {
+            for (const Request& r : t.reqs) {
+                lat_ms_.push_back(std::chrono::duration<double, std::milli>(now - r.arrival).count());
+                scene_busy_[r.slot] = 0;
+            }
+            completed_ += static_cast<int64_t>(t.reqs.size());
+            vertex_updates_ += t.vertex_updates;
+            slots_[t.stream].busy = false;
+        }
+        if (pending_.empty() && inflight_.empty()) idle_cv_.notify_all();
+    }
+}
+
+bool Server::try_launch(std::unique_lock<std::mutex>& lk) {
+    int si = -1;
+    for (size_t i = 0; i < slots_.size(); ++i)
+        if (!slots_[i].busy) {
+            si = static_cast<int>(i);
+            break;
+        }
+    if (si < 0 || pending_.empty()) return false;
+    Tick tick;
+    tick.stream = si;
+    std::vector<char> picked(scenes_.size(), 0);
+    float dt = 0.f;
+    int sub = 0;
+    std::deque<Request> rest;
+    for (Request& r : pending_) {
+        const bool fits = static_cast<int>(tick.reqs.size()) < max_batch_ && !scene_busy_[r.slot] && !picked[r.slot] &&
+                          (tick.reqs.empty() || (r.dt == dt && r.substeps == sub));
+        if (fits) {
+            if (tick.reqs.empty()) {
+                dt = r.dt;
+                sub = r.substeps;
+            }
+            picked[r.slot] = 1;
+            tick.reqs.push_back(r);
+        } else {
+            rest.push_back(r);
+        }
+    }
+    if (tick.reqs.empty()) return false;
+    pending_.swap(rest);
+    for (const Request& r : tick.reqs) scene_busy_[r.slot] = 1;
+    StreamSlot& slot = slots_[si];
+    slot.busy = true;
+    std::vector<Scene*> sc;
+    std::vector<int> idx;
+    for (const Request& r : tick.reqs) {
+        sc.push_back(scenes_[r.slot]);
+        idx.push_back(r.slot);
+    }
+    batches_++;
+    batch_sum_ += static_cast<int64_t>(tick.reqs.size());
+    max_batch_seen_ = std::max<int64_t>(max_batch_seen_, static_cast<int64_t>(tick.reqs.size()));
+    const SceneDev* table = table_.get();
+    lk.unlock();
+    // plan + launch outside the lock so that submit() is never blocked by it
+    slot.sched.build(sc, idx, tiles_);
+    slot.dev.upload(eng_, slot.sched, slot.stream, false);
+    launch_schedule(slot.sched, slot.dev, table, dt, sub, slot.stream, nullptr);
+    OE_HIP(hipEventRecord(slot.done, slot.stream));
+    tick.done = slot.done;
+    tick.vertex_updates = slot.sched.vertex_updates_per_substep * sub;
+    for (Scene* s : sc) s->steps_done++;
+    lk.lock();
+    inflight_.push_back(std::move(tick));
+    return true;
+}
+
+void Server::worker() {
+    try {
+        OE_HIP(hipSetDevice(eng_->device));
+        while (true) {
+            retire(false);
+            std::unique_lock<std::mutex> lk(mu_);
+            if (stop_) break;
+            if (try_launch(lk)) continue;
+            if (inflight_.empty()) {
+                cv_.wait(lk, [&] { return stop_.load() || !pending_.empty(); });
+            } else {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        error_ = e.what();
+        idle_cv_.notify_all();
+    }
+}
+
+Server::Stats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    Stats s;
+    s.completed = completed_;
+    s.vertex_updates = vertex_updates_;
+    s.batches = batches_;
+    s.max_batch_seen = max_batch_seen_;
+    s.mean_batch = batches_ ? static_cast<double>(batch_sum_) / batches_ : 0.0;
+    if (!lat_ms_.empty()) {
+        std::vector<double> v = lat_ms_;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            const double x = p * (v.size() - 1);
+            const size_t i = static_cast<size_t>(x);
+            const double f = x - i;
+            return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+        };
+        s.p50_ms = pct(0.50);
+        s.p99_ms = pct(0.99);
+        s.max_ms = v.back();
+    }
+    return s;
+}
+
+std::vector<double> Server::latencies() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return lat_ms_;
+}
+
+}  // namespace oe

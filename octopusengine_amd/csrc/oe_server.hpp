@@ -1,0 +1,101 @@
+// Continuous-batching request loop over HBM-resident scenes.
+//
+// There is no counterpart in the reference (it steps every scene once per
+// frame from Engine::update, src/Core/Engine.cpp:15-21).  A request asks for
+// one scene to be advanced by step(dt, substeps).  A worker thread repeatedly
+//   * retires finished ticks (HIP event query) and records each request's
+//     latency = completion time - arrival time,
+//   * forms a tick from the oldest pending requests whose scenes are not
+//     already on the device (one request per scene per tick, up to max_batch
+//     scenes, same dt/substeps),
+//   * launches the tick on a free stream as one batched step (the launch plan
+//     of oe_batch) - so requests that arrive while others run are admitted as
+//     soon as a stream slot and their scene are free, and ticks on different
+//     streams overlap on the device.
+// Scene state never leaves HBM; scenes are admitted against the engine's
+// memory budget when they are created.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "oe_batch.hpp"
+
+namespace oe {
+
+class Server {
+public:
+    Server(Engine* eng, int max_batch, int nstreams);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    int add_scene(Scene* s);                                 // returns the slot id
+    uint64_t submit(int slot, float dt, int substeps);       // thread-safe
+    void drain();                                            // wait for every submitted request
+    struct Stats {
+        int64_t completed = 0, vertex_updates = 0, batches = 0, max_batch_seen = 0;
+        double p50_ms = 0, p99_ms = 0, max_ms = 0, mean_batch = 0;
+    };
+    Stats stats();
+    std::vector<double> latencies();
+
+private:
+    using Clock = std::chrono::steady_clock;
+    struct Request {
+        uint64_t ticket;
+        int slot;
+        float dt;
+        int substeps;
+        Clock::time_point arrival;
+    };
+    struct Tick {
+        int stream = -1;
+        std::vector<Request> reqs;
+        hipEvent_t done = nullptr;
+        int64_t vertex_updates = 0;
+    };
+    struct StreamSlot {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        Schedule sched;
+        DeviceSchedule dev;
+        bool busy = false;
+    };
+
+    void worker();
+    bool try_launch(std::unique_lock<std::mutex>& lk);
+    void retire(bool block);
+    void grow_table();
+
+    Engine* eng_;
+    int max_batch_;
+    std::vector<Scene*> scenes_;
+    std::vector<DBuf<unsigned char>> flags_;
+    std::vector<SceneDev> host_table_;
+    DBuf<SceneDev> table_;
+    size_t table_uploaded_ = 0;
+    std::map<int, GatherTile> tiles_;
+    std::vector<StreamSlot> slots_;
+    std::vector<char> scene_busy_;
+    std::deque<Request> pending_;
+    std::vector<Tick> inflight_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::condition_variable idle_cv_;
+    std::atomic<bool> stop_{false};
+    uint64_t next_ticket_ = 1;
+    int64_t submitted_ = 0, completed_ = 0, vertex_updates_ = 0, batches_ = 0, batch_sum_ = 0, max_batch_seen_ = 0;
+    std::vector<double> lat_ms_;
+    std::string error_;
+    std::thread thread_;
+};
+
+}  // namespace oe

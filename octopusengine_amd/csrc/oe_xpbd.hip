@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_xpbd_v0(XpbdArgs a) {
             jx_from_nodes<S>(xs, dn, Jx);
 #pragma unroll
             for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
-            mat_mul(Jx, JXi, F);
+            deformation_gradient(Jx, JXi, F);
             xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
             scale_stress(P, JXi, rec[9], Ps);
             C += e * rec[9];
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
             }
 #pragma unroll
             for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
-            mat_mul(Jx, JXi, F);
+            deformation_gradient(Jx, JXi, F);
             xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
             scale_stress(P, JXi, rec[9], Ps);
             e_loc = e * rec[9];
