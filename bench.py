@@ -122,26 +122,28 @@ def batched_leg(args, eng) -> dict:
     wall = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
     gather_ms, launches = b.get_timing()
     b.set_timing(False)
-    # same steps from a captured HIP graph
-    b.set_graph(True)
-    b.step(DT, 1)
-    wall_graph = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
-    lat = []
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        b.step(DT, 1)
-        eng.synchronize()
-        lat.append((time.perf_counter() - t) * 1e3)
-    b.set_graph(False)
-    # plain per-scene loop on the same scenes (each scene steps on its own)
+    wall_graph, wall_plain, lat = wall, float("nan"), [float("nan")]
     plain_steps = max(1, args.plain_steps)
+    if not args.eager_only:
+        # same steps from a captured HIP graph
+        b.set_graph(True)
+        b.step(DT, 1)
+        wall_graph = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
+        lat = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            b.step(DT, 1)
+            eng.synchronize()
+            lat.append((time.perf_counter() - t) * 1e3)
+        b.set_graph(False)
 
-    def plain_step():
-        for s in scenes:
-            s.step(DT, 1)
+        # plain per-scene loop on the same scenes (each scene steps on its own)
+        def plain_step():
+            for s in scenes:
+                s.step(DT, 1)
 
-    plain_step()
-    wall_plain = timed(plain_step, plain_steps, eng.synchronize)
+        plain_step()
+        wall_plain = timed(plain_step, plain_steps, eng.synchronize)
     finite = all(bool(np.isfinite(s.positions()).all()) for s in scenes[:8])
     bytes_total = info["gather_bytes"] * args.steps
     avg_launch_s = gather_ms / 1e3 / max(launches, 1)
@@ -248,6 +250,37 @@ def cpu_xpbd_reference_leg() -> dict:
             "vertex_substeps_per_s": 425 * 20 * 100 / dt, "seconds": dt, "cores": 1}
 
 
+def copy_calibration(nbytes: int) -> dict:
+    """Device-to-device copy of nbytes with the HIP runtime (its copy kernel):
+    the reference point for scaling rocprof FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"bytes": nbytes, "seconds": dt, "gbs_read_plus_write": 2 * nbytes / dt / 1e9}
+
+
+def _finite_or_null(obj):
+    """JSON has no NaN: legs that were skipped report null."""
+    if isinstance(obj, float) and not np.isfinite(obj):
+        return None
+    if isinstance(obj, dict):
+        return {k: _finite_or_null(v) for k, v in obj.items()}
+    if isinstance(obj, list):
+        return [_finite_or_null(v) for v in obj]
+    return obj
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +293,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-single", action="store_true", help="skip the single-scene leg")
+    ap.add_argument("--eager-only", action="store_true",
+                    help="batched leg: only the eager timed region (no graph, latency or plain loop)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run a device copy of this many bytes (rocprof counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU per process: scenes shard across GPUs by scene (design only); "
@@ -299,10 +336,12 @@ def main():
     del scenes
     if not args.no_single:
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
+    if args.copy_bytes > 0:
+        out["copy_calibration"] = copy_calibration(args.copy_bytes)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_leg(args.cpu_steps)
         out["cpu_reference_xpbd_p1_16x4x4"] = cpu_xpbd_reference_leg()
-    print(json.dumps(out), flush=True)
+    print(json.dumps(_finite_or_null(out)), flush=True)
 
 
 if __name__ == "__main__":
