@@ -270,6 +270,82 @@ def copy_calibration(nbytes: int) -> dict:
     return {"bytes": nbytes, "seconds": dt, "gbs_read_plus_write": 2 * nbytes / dt / 1e9}
 
 
+def served_specs(n: int, seed: int):
+    """(element, cells, E, nu) of the served pool: vertex counts drawn
+    log-normal (median 20k, sigma 1, capped at 500k), element P2 or Q2 with
+    probability 1/2, beams of aspect 4:1:1 sized to the drawn count:
+    N = (2 nx + 1)(2 k + 1)^2 with ny = nz = k."""
+    import octopusengine_amd as oe
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        target = min(float(np.exp(np.log(20000.0) + rng.normal())), 500000.0)
+        k = max(1, int(round((target / 32.0) ** (1.0 / 3.0))))
+        nx = max(1, int(round((target / (2 * k + 1) ** 2 - 1.0) / 2.0)))
+        el = oe.TETRA10 if rng.random() < 0.5 else oe.HEXA27
+        E = float(10.0 ** rng.uniform(5.0, 7.0))
+        nu = float(rng.uniform(0.3, 0.49))
+        specs.append((el, (nx, k, k), E, nu))
+    return specs
+
+
+def served_leg(args) -> dict:
+    """Poisson step requests against a pool of resident scenes through the
+    continuous-batching server; latency = completion - submission."""
+    import octopusengine_amd as oe
+
+    eng = oe.Engine(0, budget_bytes=int(args.serve_budget_gb * 2**30))
+    specs = served_specs(args.serve_pool, args.seed + 1)
+    t0 = time.perf_counter()
+    scenes = build_scenes(eng, specs, args.workers)
+    setup_s = time.perf_counter() - t0
+    nv = np.array([s.n_vertices for s in scenes])
+    srv = oe.Server(eng, max_batch=args.serve_max_batch, streams=args.serve_streams)
+    slots = [srv.add_scene(s) for s in scenes]
+    for sl in slots:                          # warm-up: every scene once
+        srv.submit(sl, DT, 1)
+    srv.drain()
+    base = srv.stats()
+    n_lat0 = base["completed"]
+    rng = np.random.default_rng(args.seed + 2)
+    n_req = int(args.serve_rate * args.serve_seconds)
+    gaps = rng.exponential(1.0 / args.serve_rate, n_req)
+    which = rng.integers(0, len(slots), n_req)
+    arrivals = np.cumsum(gaps)
+    lag = 0.0
+    t_start = time.perf_counter()
+    for a, k in zip(arrivals, which):
+        while True:
+            now = time.perf_counter() - t_start
+            if now >= a:
+                break
+            if a - now > 1e-3:
+                time.sleep(a - now - 5e-4)
+        lag = max(lag, time.perf_counter() - t_start - a)
+        srv.submit(slots[k], DT, 1)
+    srv.drain()
+    wall = time.perf_counter() - t_start
+    st = srv.stats()
+    lat = srv.latencies()[n_lat0:]
+    vu = st["vertex_updates"] - base["vertex_updates"]
+    out = {
+        "pool_scenes": len(scenes), "pool_vertices": int(nv.sum()), "median_scene_vertices": float(np.median(nv)),
+        "max_scene_vertices": int(nv.max()), "offered_rate_req_s": args.serve_rate, "requests": n_req,
+        "seconds": args.serve_seconds, "wall_s": wall, "max_submit_lag_ms": lag * 1e3,
+        "offered_vertex_updates_per_s": float(nv[which].sum() * ITERS / max(arrivals[-1], 1e-9)),
+        "value": vu / wall, "unit": "vertex-updates/s",
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "max_ms": float(lat.max()), "ticks": st["batches"] - base["batches"], "mean_batch": st["mean_batch"],
+        "max_batch": args.serve_max_batch, "streams": args.serve_streams, "setup_s": setup_s,
+        "device_bytes": int(eng.memory()[0]), "budget_bytes": int(eng.memory()[1]),
+    }
+    srv.close()
+    for s in scenes:
+        s.close()
+    return out
+
+
 def _finite_or_null(obj):
     """JSON has no NaN: legs that were skipped report null."""
     if isinstance(obj, float) and not np.isfinite(obj):
@@ -293,6 +369,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-single", action="store_true", help="skip the single-scene leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
+    ap.add_argument("--serve-pool", type=int, default=1024, help="resident scenes in the served pool")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson request stream")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="step requests per second")
+    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-streams", type=int, default=2)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--eager-only", action="store_true",
                     help="batched leg: only the eager timed region (no graph, latency or plain loop)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -333,7 +416,11 @@ def main():
         "batched": {k: bat[k] for k in ("graph_value", "graph_ms_per_step", "plain_value", "plain_ms_per_step",
                                         "speedup_vs_plain", "setup_s", "positions_finite")},
     }
+    for sc in scenes:
+        sc.close()
     del scenes
+    if not args.no_serve:
+        out["served"] = served_leg(args)
     if not args.no_single:
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
     if args.copy_bytes > 0:

@@ -108,9 +108,18 @@ oe_status oe_scene_box_limit(oe_scene* s, const float pmin[3], const float pmax[
 oe_status oe_scene_step(oe_scene* s, float dt, int32_t substeps);
 /* Capture the substep loop into a HIP graph and replay it (1) or launch eagerly (0). */
 oe_status oe_scene_set_graph(oe_scene* s, int32_t enable);
-/* Gather tile: lanes per vertex (4..64, power of two), quadrature points per lane. */
-oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl);
-oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl);
+/* Gather tile: lanes per vertex (4..64, power of two), quadrature points per
+ * lane, and cache = 1 to keep the displacement gradients of every (element,
+ * quadrature point) in device memory during a VBD sweep (qpl must be 1; VBD
+ * scenes only).  No counterpart in the reference, whose kernels fix the
+ * thread layout per version (src/GPU/VBD/GPU_VBD_FEM.cu:712-756). */
+oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t cache);
+oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* cache);
+/* Element record layout chosen at creation: affine = 1 when every element is
+ * an affine image of the reference element and its J^-1, |det J| are stored
+ * once per element; 0 for one record per (element, quadrature point) as in
+ * the reference's GPU_FEM (src/GPU/GPU_FEM.cu:176-214). */
+oe_status oe_scene_get_layout(oe_scene* s, int32_t* affine);
 
 /* Counts: vertices, elements, nodes per element, quadrature points, colours. */
 oe_status oe_scene_info(oe_scene* s, int32_t* n_vertices, int32_t* n_elements, int32_t* nodes_per_element,
@@ -141,7 +150,7 @@ oe_status oe_batch_create(oe_engine* e, oe_scene* const* scenes, int32_t n, oe_b
 oe_status oe_batch_destroy(oe_batch* b);
 oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps);
 oe_status oe_batch_set_graph(oe_batch* b, int32_t enable);
-oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl);
+oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t cache);
 oe_status oe_batch_set_timing(oe_batch* b, int32_t enable);
 oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches);
 /* Gather launches and vertex-updates per substep; algorithmic gather bytes per substep. */
@@ -187,6 +196,13 @@ oe_status oe_host_color_dual(int32_t element, const int32_t* topology, int32_t n
 /* JX_inv (column-major) + V per (element, quadrature point): 10 floats each. */
 oe_status oe_host_fem_constants(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
                                 int32_t n_elements, float* jxv);
+/* Per-element records of an affine mesh (every element X_k = a + B xi_k up to
+ * coordinate rounding): records = 10 floats per element (B^-1 column-major,
+ * |det B|), weights = the Q quadrature weights, V_q = |det B| weights[q].
+ * *affine = 0 (outputs untouched) when some element is curved.  No counterpart
+ * in the reference, which stores one record per (element, quadrature point). */
+oe_status oe_host_affine_records(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
+                                 int32_t n_elements, float* records, float* weights, int32_t* affine);
 oe_status oe_host_masses(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
                          int32_t n_elements, float density, int32_t distribution, float* masses);
 

@@ -93,6 +93,19 @@ def host_fem_constants(element: int, xyz, topo):
     return out
 
 
+def host_affine_records(element: int, xyz, topo):
+    """(records (E, 10), weights (Q,)) of an affine mesh, or None when some
+    element is curved."""
+    xyz, topo = _f32(xyz), _i32(topo)
+    Q = host_shape(element)[1].shape[0]
+    rec = np.empty((topo.shape[0], 10), np.float32)
+    w = np.empty(Q, np.float32)
+    flag = C.c_int32()
+    L.check(L.lib.oe_host_affine_records(element, L.fptr(xyz), xyz.shape[0], L.iptr(topo), topo.shape[0],
+                                         L.fptr(rec), L.fptr(w), C.byref(flag)))
+    return (rec, w) if flag.value else None
+
+
 def host_masses(element: int, xyz, topo, density: float, distribution: int):
     xyz, topo = _f32(xyz), _i32(topo)
     out = np.empty(xyz.shape[0], np.float32)
@@ -197,13 +210,22 @@ class Scene:
     def set_graph(self, enable: bool):
         L.check(L.lib.oe_scene_set_graph(self._h, int(enable)))
 
-    def set_tile(self, lanes: int, qpl: int):
-        L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl))
+    def set_tile(self, lanes: int, qpl: int, cache: bool = False):
+        """Gather tile: lanes per vertex, quadrature points per lane, and
+        whether the VBD sweep caches the displacement gradients (qpl = 1)."""
+        L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl, int(cache)))
 
     def get_tile(self):
-        a, b = C.c_int32(), C.c_int32()
-        L.check(L.lib.oe_scene_get_tile(self._h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """(lanes, quadrature points per lane, gradient cache on)."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        L.check(L.lib.oe_scene_get_tile(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, bool(c.value)
+
+    def affine(self) -> bool:
+        """True when the element records are stored once per element."""
+        a = C.c_int32()
+        L.check(L.lib.oe_scene_get_layout(self._h, C.byref(a)))
+        return bool(a.value)
 
     def set_timing(self, enable: bool):
         L.check(L.lib.oe_scene_set_timing(self._h, int(enable)))
@@ -288,8 +310,8 @@ class Batch:
     def set_graph(self, enable: bool):
         L.check(L.lib.oe_batch_set_graph(self._h, int(enable)))
 
-    def set_tile(self, element: int, lanes: int, qpl: int):
-        L.check(L.lib.oe_batch_set_tile(self._h, element, lanes, qpl))
+    def set_tile(self, element: int, lanes: int, qpl: int, cache: bool = False):
+        L.check(L.lib.oe_batch_set_tile(self._h, element, lanes, qpl, int(cache)))
 
     def set_timing(self, enable: bool):
         L.check(L.lib.oe_batch_set_timing(self._h, int(enable)))

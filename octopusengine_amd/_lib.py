@@ -84,8 +84,9 @@ _SIGS = {
     "oe_scene_box_limit": (C.c_int32, [P, f32p, f32p]),
     "oe_scene_step": (C.c_int32, [P, C.c_float, C.c_int32]),
     "oe_scene_set_graph": (C.c_int32, [P, C.c_int32]),
-    "oe_scene_set_tile": (C.c_int32, [P, C.c_int32, C.c_int32]),
-    "oe_scene_get_tile": (C.c_int32, [P, i32p, i32p]),
+    "oe_scene_set_tile": (C.c_int32, [P, C.c_int32, C.c_int32, C.c_int32]),
+    "oe_scene_get_tile": (C.c_int32, [P, i32p, i32p, i32p]),
+    "oe_scene_get_layout": (C.c_int32, [P, i32p]),
     "oe_scene_info": (C.c_int32, [P, i32p, i32p, i32p, i32p, i32p, u64p]),
     "oe_scene_get_positions": (C.c_int32, [P, f32p]),
     "oe_scene_get_last_positions": (C.c_int32, [P, f32p]),
@@ -102,7 +103,7 @@ _SIGS = {
     "oe_batch_destroy": (C.c_int32, [P]),
     "oe_batch_step": (C.c_int32, [P, C.c_float, C.c_int32]),
     "oe_batch_set_graph": (C.c_int32, [P, C.c_int32]),
-    "oe_batch_set_tile": (C.c_int32, [P, C.c_int32, C.c_int32, C.c_int32]),
+    "oe_batch_set_tile": (C.c_int32, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "oe_batch_set_timing": (C.c_int32, [P, C.c_int32]),
     "oe_batch_get_timing": (C.c_int32, [P, f64p, i64p]),
     "oe_batch_info": (C.c_int32, [P, i32p, i64p, f64p]),
@@ -120,6 +121,7 @@ _SIGS = {
     "oe_host_color_dual": (C.c_int32, [C.c_int32, i32p, C.c_int32, C.c_int32, i32p, i32p, i64p]),
     "oe_host_fem_constants": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, f32p]),
     "oe_host_masses": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, C.c_float, C.c_int32, f32p]),
+    "oe_host_affine_records": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, f32p, f32p, i32p]),
 }
 
 OPTIONAL_SIGS: dict = {}

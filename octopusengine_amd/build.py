@@ -48,8 +48,7 @@ def _jobs() -> list[tuple[list[str], Path]]:
         jobs.append(([HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)], obj))
     for s, q in GATHER_VARIANTS:
         obj = OUT_DIR / f"oe_gather_{s}_{q}.o"
-        defs = [f"-DOE_GATHER_S={s}", f"-DOE_GATHER_Q={q}", f"-DOE_GATHER_FN=launch_gather_{s}_{q}",
-                f"-DOE_GATHER_BATCH_FN=launch_batch_gather_{s}_{q}"]
+        defs = [f"-DOE_GATHER_S={s}", f"-DOE_GATHER_Q={q}"]
         jobs.append(([HIPCC, *HIP_FLAGS, *defs, "-c", str(CSRC / "oe_gather.hip"), "-o", str(obj)], obj))
     return jobs
 

@@ -1,6 +1,7 @@
 #include "oe_batch.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 
@@ -45,9 +46,17 @@ std::vector<unsigned char> fixed_flags(const Scene& s) {
     return f;
 }
 
-SceneDev make_scene_dev(Scene& s, const unsigned char* fixed) {
+namespace {
+GatherTile tile_of(const std::map<int, GatherTile>& tiles, Element el) {
+    const auto it = tiles.find(el);
+    return it != tiles.end() ? it->second : default_tile(el);
+}
+}  // namespace
+
+SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int, GatherTile>& tiles) {
     check_batchable(s);
     SceneDev d{};
+    d.ju = tile_of(tiles, s.mesh.element).cache ? s.vbd_fem->ensure_ju() : nullptr;
     d.ps = s.ps->get_parameters();
     d.fem = s.fem->fem_view();
     d.own = OwnersView{s.vbd_fem->d_slot_vid.get(), s.vbd_fem->d_slot_nbo.get(), s.vbd_fem->d_own.get()};
@@ -80,6 +89,9 @@ void Schedule::clear() {
     glaunches.clear();
     vsegs.clear();
     vblock_seg.clear();
+    jlaunches.clear();
+    jsegs.clear();
+    jblock_seg.clear();
     max_iterations = 0;
     vertex_updates_per_substep = 0;
     algorithmic_bytes_per_substep = 0.0;
@@ -107,8 +119,7 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
             for (size_t i = 0; i < scenes.size(); ++i) {
                 Scene* s = scenes[i];
                 if (s->mesh.element != el || c >= s->vbd_fem->nb_colors()) continue;
-                const auto it = tiles.find(el);
-                const GatherTile t = it != tiles.end() ? it->second : default_tile(el);
+                const GatherTile t = tile_of(tiles, el);
                 const int first = s->vbd_fem->owners.color_begin[c];
                 const int count = s->vbd_fem->owners.color_begin[c + 1] - first;
                 if (count <= 0) continue;
@@ -117,6 +128,7 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     gl.Q = s->fem->Q;
                     gl.L = t.L;
                     gl.QL = t.QL;
+                    gl.cache = t.cache;
                     any = true;
                 }
                 const int nb = static_cast<int>((static_cast<int64_t>(count) * gl.L + kBlock - 1) / kBlock);
@@ -137,6 +149,28 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
         vsegs.push_back(VSeg{table_index[i], n, static_cast<int>(vblock_seg.size())});
         vblock_seg.insert(vblock_seg.end(), nb, sid);
     }
+    for (Element el : order) {
+        if (!tile_of(tiles, el).cache) continue;
+        JLaunch jl{};
+        jl.block_base = static_cast<int>(jblock_seg.size());
+        bool any = false;
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            Scene* s = scenes[i];
+            if (s->mesh.element != el) continue;
+            const int64_t recs = static_cast<int64_t>(s->fem->E) * s->fem->Q;
+            if (recs <= 0) continue;
+            if (recs > INT32_MAX) throw std::invalid_argument("scene too large for the gradient cache");
+            jl.S = s->fem->S;
+            jl.Q = s->fem->Q;
+            any = true;
+            const int nb = static_cast<int>((recs + kBlock - 1) / kBlock);
+            const int sid = static_cast<int>(jsegs.size());
+            jsegs.push_back(VSeg{table_index[i], static_cast<int>(recs), static_cast<int>(jblock_seg.size())});
+            jblock_seg.insert(jblock_seg.end(), nb, sid);
+        }
+        jl.nblocks = static_cast<int>(jblock_seg.size()) - jl.block_base;
+        if (any) jlaunches.push_back(jl);
+    }
 }
 
 namespace {
@@ -152,6 +186,8 @@ void DeviceSchedule::upload(Engine* eng, const Schedule& s, hipStream_t stream, 
     upload_grow(eng, block_seg, s.block_seg, stream);
     upload_grow(eng, vsegs, s.vsegs, stream);
     upload_grow(eng, vblock_seg, s.vblock_seg, stream);
+    upload_grow(eng, jsegs, s.jsegs, stream);
+    upload_grow(eng, jblock_seg, s.jblock_seg, stream);
     if (sync) OE_HIP(hipStreamSynchronize(stream));
 }
 
@@ -194,14 +230,21 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
     BatchVArgs v{d_scenes, dev.vsegs.get(), dev.vblock_seg.get(), h, 0};
     const int vblocks = static_cast<int>(sch.vblock_seg.size());
     BatchGatherArgs g{d_scenes, dev.segs.get(), dev.block_seg.get(), 0, 0, h};
+    BatchRefreshArgs r{d_scenes, dev.jsegs.get(), dev.jblock_seg.get(), 0, 0};
     for (int sub = 0; sub < substeps; ++sub) {
         launch_batch_predict(v, vblocks, stream);
         for (int it = 0; it < sch.max_iterations; ++it) {
             g.iteration = it;
+            r.iteration = it;
+            for (const Schedule::JLaunch& jl : sch.jlaunches) {
+                r.block_base = jl.block_base;
+                if (!launch_batch_ju_refresh(jl.S, jl.Q, r, jl.nblocks, stream))
+                    throw std::invalid_argument("unsupported element in batch");
+            }
             for (const Schedule::GLaunch& gl : sch.glaunches) {
                 g.block_base = gl.block_base;
                 if (timer) timer->before(stream);
-                if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, g, gl.nblocks, stream))
+                if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.cache, g, gl.nblocks, stream))
                     throw std::invalid_argument("unsupported gather tile in batch");
                 if (timer) timer->after(stream);
             }
@@ -219,13 +262,9 @@ Batch::Batch(Engine* e, std::vector<Scene*> sc) : eng(e), scenes(std::move(sc)),
         if (s->eng != eng) throw std::invalid_argument("scene belongs to another engine");
         check_batchable(*s);
     }
-    std::vector<SceneDev> host;
     flags.reserve(scenes.size());
-    for (Scene* s : scenes) {
-        flags.emplace_back(&eng->pool, fixed_flags(*s));
-        host.push_back(make_scene_dev(*s, flags.back().get()));
-    }
-    d_scenes = DBuf<SceneDev>(&eng->pool, host);
+    for (Scene* s : scenes) flags.emplace_back(&eng->pool, fixed_flags(*s));
+    d_scenes.allocate(&eng->pool, scenes.size());
     rebuild();
 }
 
@@ -233,6 +272,9 @@ Batch::~Batch() { drop_graph(); }
 
 void Batch::rebuild() {
     drop_graph();
+    std::vector<SceneDev> host;
+    for (size_t i = 0; i < scenes.size(); ++i) host.push_back(make_scene_dev(*scenes[i], flags[i].get(), tiles));
+    d_scenes.upload(host, eng->stream);
     std::vector<int> idx(scenes.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = static_cast<int>(i);
     sched.build(scenes, idx, tiles);

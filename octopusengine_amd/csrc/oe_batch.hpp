@@ -7,7 +7,9 @@
 // colour c of scene B can be solved concurrently, and a launch carries enough
 // vertices to fill the chip.  Per step the launch sequence is
 //   predictor (all vertices of all scenes)
-//   for each iteration:  for each global colour c, for each element type:
+//   for each iteration:  refresh of the cached displacement gradients (per
+//                          element type whose tile caches them)
+//                        for each global colour c, for each element type:
 //                          gather + local solve of colour c
 //                        fixation + box limit (all vertices, one fused kernel)
 //   velocity update
@@ -29,15 +31,24 @@ namespace oe {
 struct Schedule {
     struct GLaunch {
         int color, S, Q, L, QL;
+        bool cache;
         int block_base, nblocks;
         int64_t slots;   // vertex slots covered (all scenes)
         int64_t incidences;
+    };
+    // refresh of cached gradients: one launch per element type, over jsegs
+    struct JLaunch {
+        int S, Q;
+        int block_base, nblocks;
     };
     std::vector<BatchSeg> segs;
     std::vector<int> block_seg;
     std::vector<GLaunch> glaunches;   // ascending colour, then element type
     std::vector<VSeg> vsegs;
     std::vector<int> vblock_seg;
+    std::vector<JLaunch> jlaunches;
+    std::vector<VSeg> jsegs;          // count = E * Q records of the scene
+    std::vector<int> jblock_seg;
     int max_iterations = 0;
     int64_t vertex_updates_per_substep = 0;
     double algorithmic_bytes_per_substep = 0.0;   // gather kernels, all iterations
@@ -54,12 +65,15 @@ struct DeviceSchedule {
     DBuf<int> block_seg;
     DBuf<VSeg> vsegs;
     DBuf<int> vblock_seg;
+    DBuf<VSeg> jsegs;
+    DBuf<int> jblock_seg;
     void upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync);
 };
 
 // Validates that a scene can be stepped in a batch and describes it for the
-// batched kernels.  `fixed` is the scene's per-vertex fixation flag array.
-SceneDev make_scene_dev(Scene& s, const unsigned char* fixed);
+// batched kernels.  `fixed` is the scene's per-vertex fixation flag array; the
+// scene's gradient cache is allocated when its element type's tile uses it.
+SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int, GatherTile>& tiles);
 std::vector<unsigned char> fixed_flags(const Scene& s);
 void check_batchable(const Scene& s);
 
@@ -92,7 +106,7 @@ struct Batch {
     Engine* eng;
     std::vector<Scene*> scenes;
     std::vector<DBuf<unsigned char>> flags;
-    DBuf<SceneDev> d_scenes;
+    DBuf<SceneDev> d_scenes;   // rewritten by rebuild()
     std::map<int, GatherTile> tiles;
     Schedule sched;
     DeviceSchedule dsched;

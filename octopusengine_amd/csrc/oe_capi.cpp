@@ -221,29 +221,45 @@ oe_status oe_scene_set_graph(oe_scene* s, int32_t enable) {
     });
 }
 
-oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl) {
+namespace {
+void check_tile(int Q, int32_t lanes, int32_t qpl, int32_t cache) {
+    OE_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "lanes must be 4..64, power of 2");
+    OE_REQUIRE(qpl >= 1 && Q % qpl == 0, "quadrature points per lane must divide Q");
+    OE_REQUIRE(qpl == 1 || qpl == Q || (Q == 4 && qpl == 2) || (Q == 27 && (qpl == 3 || qpl == 9)),
+               "tile not instantiated");
+    OE_REQUIRE(cache == 0 || cache == 1, "cache must be 0 or 1");
+    OE_REQUIRE(cache == 0 || qpl == 1, "the gradient cache takes one quadrature point per lane");
+}
+}  // namespace
+
+oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t cache) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
         oe::Scene& sc = *s->scene;
-        OE_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "lanes must be 4..64, power of 2");
-        const int Q = sc.fem->Q;
-        OE_REQUIRE(qpl >= 1 && Q % qpl == 0, "quadrature points per lane must divide Q");
-        OE_REQUIRE(qpl == 1 || qpl == Q || (Q == 4 && qpl == 2) || (Q == 27 && (qpl == 3 || qpl == 9)),
-                   "tile not instantiated");
+        check_tile(sc.fem->Q, lanes, qpl, cache);
+        OE_REQUIRE(cache == 0 || sc.vbd_fem, "the gradient cache is a VBD option");
         sc.set_graph(false);
-        if (sc.vbd_fem) sc.vbd_fem->tile = oe::GatherTile{lanes, qpl};
-        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl};
+        if (sc.vbd_fem) sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, cache != 0};
+        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, false};
     });
 }
 
-oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl) {
+oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* cache) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
-        oe::GatherTile t{0, 0};
+        oe::GatherTile t{0, 0, false};
         if (s->scene->vbd_fem) t = s->scene->vbd_fem->tile;
         if (s->scene->explicit_fem) t = s->scene->explicit_fem->tile;
         if (lanes) *lanes = t.L;
         if (qpl) *qpl = t.QL;
+        if (cache) *cache = (s->scene->vbd_fem && s->scene->vbd_fem->cached()) ? 1 : 0;
+    });
+}
+
+oe_status oe_scene_get_layout(oe_scene* s, int32_t* affine) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        if (affine) *affine = s->scene->fem->affine ? 1 : 0;
     });
 }
 
@@ -377,15 +393,11 @@ oe_status oe_batch_set_graph(oe_batch* b, int32_t enable) {
     });
 }
 
-oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl) {
+oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t cache) {
     return guard([&] {
         OE_REQUIRE(b && oe::element_supported(element), "bad argument");
-        OE_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "lanes must be 4..64, power of 2");
-        const int Q = oe::elem_nb_quadrature(static_cast<oe::Element>(element));
-        OE_REQUIRE(qpl >= 1 && Q % qpl == 0, "quadrature points per lane must divide Q");
-        OE_REQUIRE(qpl == 1 || qpl == Q || (Q == 4 && qpl == 2) || (Q == 27 && (qpl == 3 || qpl == 9)),
-                   "tile not instantiated");
-        b->batch->set_tile(static_cast<oe::Element>(element), oe::GatherTile{lanes, qpl});
+        check_tile(oe::elem_nb_quadrature(static_cast<oe::Element>(element)), lanes, qpl, cache);
+        b->batch->set_tile(static_cast<oe::Element>(element), oe::GatherTile{lanes, qpl, cache != 0});
     });
 }
 
@@ -549,6 +561,18 @@ oe_status oe_host_fem_constants(int32_t element, const float* xyz, int32_t nv, c
         const oe::HostMesh m = mesh_from(element, xyz, nv, topology, ne);
         const oe::FemConstants fc = oe::build_fem_constants(m);
         if (jxv) std::memcpy(jxv, fc.jxv.data(), fc.jxv.size() * sizeof(float));
+    });
+}
+
+oe_status oe_host_affine_records(int32_t element, const float* xyz, int32_t nv, const int32_t* topology, int32_t ne,
+                                 float* records, float* weights, int32_t* affine) {
+    return guard([&] {
+        OE_REQUIRE(affine, "bad argument");
+        const oe::HostMesh m = mesh_from(element, xyz, nv, topology, ne);
+        std::vector<float> rec, wq;
+        *affine = oe::build_affine_records(m, rec, wq) ? 1 : 0;
+        if (*affine && records) std::memcpy(records, rec.data(), rec.size() * sizeof(float));
+        if (*affine && weights) std::memcpy(weights, wq.data(), wq.size() * sizeof(float));
     });
 }
 

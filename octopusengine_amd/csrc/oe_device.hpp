@@ -144,12 +144,33 @@ struct PSView {
 };
 
 // Per-mesh FEM data (reference GPU_FEM_Pameters, include/GPU/GPU_FEM.h:49-59).
+// Two layouts of the element records:
+//   affine == 0: E * Q * 10 floats, JX_inv (column-major) then V = |det J_q| w_q
+//                for every (element, quadrature point);
+//   affine == 1: E * 10 floats, JX_inv then |det J| once per element, and
+//                V_q = |det J| * wq[q].  Used when every element's map from
+//                the reference element is affine (straight-sided tetrahedra,
+//                parallelepiped hexahedra), where J does not depend on q.
+// wq holds the quadrature weights in the affine layout and ones otherwise, so
+// that V = rec[9] * wq[q] in both.
 struct FemView {
     int E, S, Q;
     const int* topo;     // E * S
-    const float* jxv;    // E * Q * 10 : JX_inv column-major, then V
+    const float* jxv;
     const float* dN;     // Q * S * 3
+    int affine;
+    const float* wq;     // Q
 };
+
+// Element record of quadrature point q of element e in either layout
+// (branch-free: the layout only changes the record index).
+__device__ __forceinline__ void load_element_record(const FemView& f, int e, int q, float JXi[9], float& V) {
+    const size_t idx = f.affine ? static_cast<size_t>(e) : static_cast<size_t>(e) * f.Q + q;
+    const float* __restrict__ r = f.jxv + idx * 10;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) JXi[i] = r[i];
+    V = r[9] * f.wq[q];
+}
 
 // Colour-major owners CSR (reference GPU_Owners_Parameters,
 // include/GPU/VBD/GPU_VBD_FEM.h:36-42).

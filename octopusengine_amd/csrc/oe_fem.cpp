@@ -10,6 +10,7 @@
 // index.
 #include "oe_host.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -59,6 +60,89 @@ FemConstants build_fem_constants(const HostMesh& m) {
         }
     }
     return fc;
+}
+
+bool build_affine_records(const HostMesh& m, std::vector<float>& rec, std::vector<float>& wq) {
+    const ShapeTable& st = shape_table(m.element);
+    const int S = st.S, Q = st.Q, E = m.nb_elements();
+    // Least-squares affine fit X ~ a + B xi over the element's nodes: the
+    // normal matrix M = sum_k phi_k phi_k^T, phi = (1, xi), is the same for
+    // every element; invert it once (Gauss-Jordan, partial pivoting).
+    double M[4][8] = {};
+    for (int k = 0; k < S; ++k) {
+        const double phi[4] = {1.0, st.nodes[3 * k], st.nodes[3 * k + 1], st.nodes[3 * k + 2]};
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) M[i][j] += phi[i] * phi[j];
+    }
+    for (int i = 0; i < 4; ++i) M[i][4 + i] = 1.0;
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(M[r][c]) > std::fabs(M[piv][c])) piv = r;
+        for (int j = 0; j < 8; ++j) std::swap(M[c][j], M[piv][j]);
+        const double d = M[c][c];
+        for (int j = 0; j < 8; ++j) M[c][j] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c];
+            for (int j = 0; j < 8; ++j) M[r][j] -= f * M[c][j];
+        }
+    }
+    std::vector<float> out(static_cast<size_t>(E) * 10);
+    for (int e = 0; e < E; ++e) {
+        const int32_t* t = &m.topo[static_cast<size_t>(e) * S];
+        double rhs[3][4] = {};
+        double xmax = 0.0;
+        for (int k = 0; k < S; ++k) {
+            const double phi[4] = {1.0, st.nodes[3 * k], st.nodes[3 * k + 1], st.nodes[3 * k + 2]};
+            const float* X = &m.xyz[3 * static_cast<size_t>(t[k])];
+            for (int i = 0; i < 3; ++i) {
+                xmax = std::max(xmax, std::fabs(static_cast<double>(X[i])));
+                for (int j = 0; j < 4; ++j) rhs[i][j] += phi[j] * X[i];
+            }
+        }
+        double c[3][4];   // c[i] = M^-1 rhs[i]: (a_i, B_i0, B_i1, B_i2)
+        for (int i = 0; i < 3; ++i)
+            for (int r = 0; r < 4; ++r) {
+                c[i][r] = 0.0;
+                for (int j = 0; j < 4; ++j) c[i][r] += M[r][4 + j] * rhs[i][j];
+            }
+        const double tol = 8.0 * std::ldexp(std::max(xmax, 1e-30), -24);
+        for (int k = 0; k < S; ++k) {
+            const float* X = &m.xyz[3 * static_cast<size_t>(t[k])];
+            for (int i = 0; i < 3; ++i) {
+                const double fit = c[i][0] + c[i][1] * st.nodes[3 * k] + c[i][2] * st.nodes[3 * k + 1] +
+                                   c[i][3] * st.nodes[3 * k + 2];
+                if (std::fabs(fit - X[i]) > tol) return false;
+            }
+        }
+        double B[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) B[i][j] = c[i][1 + j];
+        const double c00 = B[1][1] * B[2][2] - B[1][2] * B[2][1];
+        const double c01 = B[1][2] * B[2][0] - B[1][0] * B[2][2];
+        const double c02 = B[1][0] * B[2][1] - B[1][1] * B[2][0];
+        const double det = B[0][0] * c00 + B[0][1] * c01 + B[0][2] * c02;
+        if (det == 0.0) throw std::invalid_argument("degenerate element (det J = 0)");
+        double inv[3][3];
+        inv[0][0] = c00 / det;
+        inv[1][0] = c01 / det;
+        inv[2][0] = c02 / det;
+        inv[0][1] = (B[0][2] * B[2][1] - B[0][1] * B[2][2]) / det;
+        inv[1][1] = (B[0][0] * B[2][2] - B[0][2] * B[2][0]) / det;
+        inv[2][1] = (B[0][1] * B[2][0] - B[0][0] * B[2][1]) / det;
+        inv[0][2] = (B[0][1] * B[1][2] - B[0][2] * B[1][1]) / det;
+        inv[1][2] = (B[0][2] * B[1][0] - B[0][0] * B[1][2]) / det;
+        inv[2][2] = (B[0][0] * B[1][1] - B[0][1] * B[1][0]) / det;
+        float* r = &out[static_cast<size_t>(e) * 10];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) r[j * 3 + i] = static_cast<float>(inv[i][j]);   // column-major
+        r[9] = static_cast<float>(std::fabs(det));
+    }
+    rec = std::move(out);
+    wq.resize(Q);
+    for (int q = 0; q < Q; ++q) wq[q] = static_cast<float>(st.weights[q]);
+    return true;
 }
 
 std::vector<double> element_volumes(const HostMesh& m) {

@@ -11,8 +11,17 @@
 // for its QL quadrature points.  Each lane accumulates the force and the
 // 3x3 Hessian block in registers; the group then reduces with cross-lane
 // shuffles (no LDS, no barrier) and its first lane solves the 3x3 system in
-// double precision.  L and QL are tile parameters chosen per element type by
-// the autotuner.
+// double precision.  L and QL are tile parameters chosen per element type.
+//
+// Cached form (CACHED = true, one quadrature point per work item): the
+// displacement gradient Ju(e, q) = sum_k u_k (x) dN_k(q) is read from a
+// per-(element, quadrature point) array instead of being rebuilt from the S
+// node positions (S loads and 9 S FMAs per point), and after the solve the
+// group adds du (x) dN_r(q) to the records of the vertex's elements.  The
+// array is rebuilt from the positions at the start of every iteration
+// (k_ju_refresh), after the predictor, constraints and any other position
+// change.  Within one colour each element has at most one vertex being
+// solved, so each record has a single reader-writer per launch.
 //
 // This file is compiled once per element type (OE_GATHER_S / OE_GATHER_Q set
 // on the command line) so that the template instances build in parallel.
@@ -22,8 +31,9 @@
 namespace oe {
 
 // Gather + local solve of one vertex slot by a group of L lanes.
-template <int S, int Q, int L, int QL>
+template <int S, int Q, int L, int QL, bool CACHED>
 __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int lane) {
+    static_assert(!CACHED || QL == 1, "cached gradients take one quadrature point per work item");
     constexpr int QC = Q / QL;
     if (grp >= a.count) return;                 // whole group leaves together
     const int slot = a.first_slot + grp;
@@ -40,25 +50,36 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
         const int pk = a.own.own[nbo.y + j];
         const int eid = pk >> 5;
         const int r = pk & 31;
-        const int* __restrict__ tp = a.fem.topo + static_cast<size_t>(eid) * S;
-        float xs[S][3];
+        if constexpr (CACHED) {
+            const int q = qc;
+            const float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
+            float Ju[9], JXi[9], V;
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const float4 x = P[tp[k]];
-            xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
-        }
+            for (int i = 0; i < 9; ++i) Ju[i] = g[i];
+            load_element_record(a.fem, eid, q, JXi, V);
+            nh_contrib(Ju, JXi, V, a.fem.dN + (q * S + r) * 3, a.mt, acc);
+        } else {
+            const int* __restrict__ tp = a.fem.topo + static_cast<size_t>(eid) * S;
+            float xs[S][3];
 #pragma unroll
-        for (int qq = 0; qq < QL; ++qq) {
-            const int q = qc * QL + qq;
-            const float* __restrict__ dn = a.fem.dN + q * S * 3;
-            float Jx[9];
-            jx_from_nodes<S>(xs, dn, Jx);
-            nh_contrib(Jx, a.fem.jxv + (static_cast<size_t>(eid) * Q + q) * 10, dn + 3 * r, a.mt, acc);
+            for (int k = 0; k < S; ++k) {
+                const float4 x = P[tp[k]];
+                xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+            }
+#pragma unroll
+            for (int qq = 0; qq < QL; ++qq) {
+                const int q = qc * QL + qq;
+                const float* __restrict__ dn = a.fem.dN + q * S * 3;
+                float Ju[9], JXi[9], V;
+                load_element_record(a.fem, eid, q, JXi, V);
+                jx_from_nodes<S>(xs, dn, Ju);
+                nh_contrib(Ju, JXi, V, dn + 3 * r, a.mt, acc);
+            }
         }
     }
     group_sum<L>(acc);
-    if (lane != 0) return;
     if (a.mode == GATHER_EXPLICIT_FORCE) {
+        if (lane != 0) return;
         // f = sum(-V P g) - k H v  (damping uses the vertex's own block)
         float4 f = make_float4(acc.v[0], acc.v[1], acc.v[2], 0.f);
         if (a.damping != 0.f) {
@@ -70,29 +91,65 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
         a.ps.f[vid] = f;
         return;
     }
-    const float4 pv = P[vid];
-    double dx[3];
-    vbd_local_solve(acc, pv, a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt, dx);
-    const float4 out = make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
-                                   pv.z + static_cast<float>(dx[2]), pv.w);
-    if (a.mode == GATHER_VBD_JACOBI)
-        a.p_new[vid] = out;
-    else
-        a.ps.p[vid] = out;
+    float du[3] = {0.f, 0.f, 0.f};
+    if (lane == 0) {
+        const float4 pv = P[vid];
+        double dx[3];
+        vbd_local_solve(acc, pv, a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt, dx);
+        const float4 out = make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
+                                       pv.z + static_cast<float>(dx[2]), pv.w);
+        if (a.mode == GATHER_VBD_JACOBI)
+            a.p_new[vid] = out;
+        else
+            a.ps.p[vid] = out;
+        du[0] = out.x - pv.x;
+        du[1] = out.y - pv.y;
+        du[2] = out.z - pv.z;
+    }
+    if constexpr (CACHED) {
+        // broadcast the move from the group's first lane, then update the
+        // gradients of every (element, quadrature point) the group read
+#pragma unroll
+        for (int k = 0; k < 3; ++k) du[k] = __shfl(du[k], 0, L);
+        if (du[0] == 0.f && du[1] == 0.f && du[2] == 0.f) return;
+        for (int t = lane; t < nwork; t += L) {
+            const int j = t / Q;
+            const int q = t - j * Q;
+            const int pk = a.own.own[nbo.y + j];
+            const int eid = pk >> 5;
+            const int r = pk & 31;
+            const float* __restrict__ dn = a.fem.dN + (q * S + r) * 3;
+            const float d0 = dn[0], d1 = dn[1], d2 = dn[2];
+            float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
+            // column-major: Ju[c*3 + k] += du[k] dN_r[c]
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                g[k] = fmaf(du[k], d0, g[k]);
+                g[3 + k] = fmaf(du[k], d1, g[3 + k]);
+                g[6 + k] = fmaf(du[k], d2, g[6 + k]);
+            }
+        }
+    }
 }
 
-template <int S, int Q, int L, int QL>
-__global__ __launch_bounds__(256) void k_vertex_gather(GatherArgs a) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    gather_body<S, Q, L, QL>(a, gt / L, gt & (L - 1));
+// One quadrature point per work item keeps the live state small; ask for at
+// least 4 waves per SIMD there so that the scheduler does not trade occupancy
+// for early issue of all S node loads.
+template <int QL>
+constexpr int gather_min_waves() { return QL == 1 ? 4 : 1; }
+
+template <int S, int Q, int L, int QL, bool CACHED>
+__global__ __launch_bounds__(256, gather_min_waves<QL>()) void k_vertex_gather(GatherArgs a) {
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    gather_body<S, Q, L, QL, CACHED>(a, gt / L, gt & (L - 1));
 }
 
 // Batched form: one launch covers one colour of many scenes.  Every block
 // belongs to one (scene, colour) segment, so the scene's description is read
 // once per block with scalar loads.
-template <int S, int Q, int L, int QL>
-__global__ __launch_bounds__(256) void k_batch_gather(BatchGatherArgs b) {
-    const int blk = b.block_base + blockIdx.x;
+template <int S, int Q, int L, int QL, bool CACHED>
+__global__ __launch_bounds__(256, gather_min_waves<QL>()) void k_batch_gather(BatchGatherArgs b) {
+    const int blk = b.block_base + xcd_block(blockIdx.x, gridDim.x);
     const BatchSeg seg = b.segs[b.block_seg[blk]];
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;
@@ -103,52 +160,90 @@ __global__ __launch_bounds__(256) void k_batch_gather(BatchGatherArgs b) {
     a.mt = sd.mt;
     a.y = sd.y;
     a.p_new = nullptr;
+    a.ju = sd.ju;
     a.damping = sd.damping;
     a.dt = b.dt;
     a.first_slot = seg.first;
     a.count = seg.count;
     a.mode = GATHER_VBD_INPLACE;
     const int local = (blk - seg.block_begin) * 256 + threadIdx.x;
-    gather_body<S, Q, L, QL>(a, local / L, local & (L - 1));
+    gather_body<S, Q, L, QL, CACHED>(a, local / L, local & (L - 1));
+}
+
+// Ju(e, q) = sum_k u_k (x) dN_k(q) for record idx = e * Q + q.
+template <int S, int Q>
+__device__ __forceinline__ void ju_refresh_one(const float4* __restrict__ P, const FemView& f, float* ju, int idx) {
+    const int e = idx / Q;
+    const int q = idx - e * Q;
+    const int* __restrict__ tp = f.topo + static_cast<size_t>(e) * S;
+    float xs[S][3];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const float4 x = P[tp[k]];
+        xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+    }
+    float Ju[9];
+    jx_from_nodes<S>(xs, f.dN + q * S * 3, Ju);
+    float* g = ju + static_cast<size_t>(idx) * 9;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = Ju[i];
+}
+
+template <int S, int Q>
+__global__ __launch_bounds__(256) void k_ju_refresh(PSView ps, FemView f, float* ju) {
+    const int idx = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (idx >= f.E * Q) return;
+    ju_refresh_one<S, Q>(ps.p, f, ju, idx);
+}
+
+template <int S, int Q>
+__global__ __launch_bounds__(256) void k_batch_ju_refresh(BatchRefreshArgs b) {
+    const int blk = b.block_base + xcd_block(blockIdx.x, gridDim.x);
+    const VSeg seg = b.segs[b.block_seg[blk]];
+    const SceneDev& sd = b.scenes[seg.scene];
+    if (b.iteration >= sd.iterations) return;
+    const int idx = (blk - seg.block_begin) * 256 + threadIdx.x;
+    if (idx >= seg.count) return;
+    ju_refresh_one<S, Q>(sd.ps.p, sd.fem, sd.ju, idx);
 }
 
 namespace {
 
-template <int S, int Q, int L, int QL>
+template <int S, int Q, int L, int QL, bool CACHED>
 void launch_one(const GatherArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
     const long long threads = static_cast<long long>(a.count) * L;
     const int block = 256;
     const int grid = static_cast<int>((threads + block - 1) / block);
-    hipLaunchKernelGGL((k_vertex_gather<S, Q, L, QL>), dim3(grid), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((k_vertex_gather<S, Q, L, QL, CACHED>), dim3(grid), dim3(block), 0, s, a);
 }
 
-template <int S, int Q, int L, int QL>
+template <int S, int Q, int L, int QL, bool CACHED>
 void launch_batch_one(const BatchGatherArgs& b, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL((k_batch_gather<S, Q, L, QL>), dim3(nblocks), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k_batch_gather<S, Q, L, QL, CACHED>), dim3(nblocks), dim3(256), 0, s, b);
 }
 
-template <int S, int Q, int QL>
+template <int S, int Q, int QL, bool CACHED>
 bool launch_batch_L(int L, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
     switch (L) {
-        case 4: launch_batch_one<S, Q, 4, QL>(b, nblocks, s); return true;
-        case 8: launch_batch_one<S, Q, 8, QL>(b, nblocks, s); return true;
-        case 16: launch_batch_one<S, Q, 16, QL>(b, nblocks, s); return true;
-        case 32: launch_batch_one<S, Q, 32, QL>(b, nblocks, s); return true;
-        case 64: launch_batch_one<S, Q, 64, QL>(b, nblocks, s); return true;
+        case 4: launch_batch_one<S, Q, 4, QL, CACHED>(b, nblocks, s); return true;
+        case 8: launch_batch_one<S, Q, 8, QL, CACHED>(b, nblocks, s); return true;
+        case 16: launch_batch_one<S, Q, 16, QL, CACHED>(b, nblocks, s); return true;
+        case 32: launch_batch_one<S, Q, 32, QL, CACHED>(b, nblocks, s); return true;
+        case 64: launch_batch_one<S, Q, 64, QL, CACHED>(b, nblocks, s); return true;
         default: return false;
     }
 }
 
-template <int S, int Q, int QL>
+template <int S, int Q, int QL, bool CACHED>
 bool launch_L(int L, const GatherArgs& a, hipStream_t s) {
     switch (L) {
-        case 4: launch_one<S, Q, 4, QL>(a, s); return true;
-        case 8: launch_one<S, Q, 8, QL>(a, s); return true;
-        case 16: launch_one<S, Q, 16, QL>(a, s); return true;
-        case 32: launch_one<S, Q, 32, QL>(a, s); return true;
-        case 64: launch_one<S, Q, 64, QL>(a, s); return true;
+        case 4: launch_one<S, Q, 4, QL, CACHED>(a, s); return true;
+        case 8: launch_one<S, Q, 8, QL, CACHED>(a, s); return true;
+        case 16: launch_one<S, Q, 16, QL, CACHED>(a, s); return true;
+        case 32: launch_one<S, Q, 32, QL, CACHED>(a, s); return true;
+        case 64: launch_one<S, Q, 64, QL, CACHED>(a, s); return true;
         default: return false;
     }
 }
@@ -156,34 +251,54 @@ bool launch_L(int L, const GatherArgs& a, hipStream_t s) {
 }  // namespace
 
 #if defined(OE_GATHER_S) && defined(OE_GATHER_Q)
+// OE_FN(launch_gather_) -> launch_gather_<S>_<Q>
+#define OE_CAT3(p, s, q) p##s##_##q
+#define OE_NAME(p, s, q) OE_CAT3(p, s, q)
+#define OE_FN(p) OE_NAME(p, OE_GATHER_S, OE_GATHER_Q)
+
 // Quadrature points per lane that divide Q: 1, Q, and for Q = 4 / 27 the
-// intermediate divisors.
-bool OE_GATHER_FN(int L, int QL, const GatherArgs& a, hipStream_t s) {
+// intermediate divisors.  The cached form takes QL = 1 only.
+bool OE_FN(launch_gather_)(int L, int QL, const GatherArgs& a, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
-    if (QL == 1) return launch_L<S, Q, 1>(L, a, s);
-    if (QL == Q) return launch_L<S, Q, Q>(L, a, s);
+    if (a.ju) return QL == 1 && a.mode == GATHER_VBD_INPLACE && launch_L<S, Q, 1, true>(L, a, s);
+    if (QL == 1) return launch_L<S, Q, 1, false>(L, a, s);
+    if (QL == Q) return launch_L<S, Q, Q, false>(L, a, s);
 #if OE_GATHER_Q == 4
-    if (QL == 2) return launch_L<S, Q, 2>(L, a, s);
+    if (QL == 2) return launch_L<S, Q, 2, false>(L, a, s);
 #endif
 #if OE_GATHER_Q == 27
-    if (QL == 3) return launch_L<S, Q, 3>(L, a, s);
-    if (QL == 9) return launch_L<S, Q, 9>(L, a, s);
+    if (QL == 3) return launch_L<S, Q, 3, false>(L, a, s);
+    if (QL == 9) return launch_L<S, Q, 9, false>(L, a, s);
 #endif
     return false;
 }
 
-bool OE_GATHER_BATCH_FN(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
+bool OE_FN(launch_batch_gather_)(int L, int QL, bool cached, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
-    if (QL == 1) return launch_batch_L<S, Q, 1>(L, b, nblocks, s);
-    if (QL == Q) return launch_batch_L<S, Q, Q>(L, b, nblocks, s);
+    if (cached) return QL == 1 && launch_batch_L<S, Q, 1, true>(L, b, nblocks, s);
+    if (QL == 1) return launch_batch_L<S, Q, 1, false>(L, b, nblocks, s);
+    if (QL == Q) return launch_batch_L<S, Q, Q, false>(L, b, nblocks, s);
 #if OE_GATHER_Q == 4
-    if (QL == 2) return launch_batch_L<S, Q, 2>(L, b, nblocks, s);
+    if (QL == 2) return launch_batch_L<S, Q, 2, false>(L, b, nblocks, s);
 #endif
 #if OE_GATHER_Q == 27
-    if (QL == 3) return launch_batch_L<S, Q, 3>(L, b, nblocks, s);
-    if (QL == 9) return launch_batch_L<S, Q, 9>(L, b, nblocks, s);
+    if (QL == 3) return launch_batch_L<S, Q, 3, false>(L, b, nblocks, s);
+    if (QL == 9) return launch_batch_L<S, Q, 9, false>(L, b, nblocks, s);
 #endif
     return false;
+}
+
+void OE_FN(launch_ju_refresh_)(const PSView& ps, const FemView& f, float* ju, hipStream_t s) {
+    constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
+    const long long n = static_cast<long long>(f.E) * Q;
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_ju_refresh<S, Q>), dim3(static_cast<int>((n + 255) / 256)), dim3(256), 0, s, ps, f, ju);
+}
+
+void OE_FN(launch_batch_ju_refresh_)(const BatchRefreshArgs& b, int nblocks, hipStream_t s) {
+    constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL((k_batch_ju_refresh<S, Q>), dim3(nblocks), dim3(256), 0, s, b);
 }
 #endif
 

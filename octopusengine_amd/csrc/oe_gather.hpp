@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "oe_device.hpp"
+#include "oe_elementwise.hpp"
 
 namespace oe {
 
@@ -20,6 +21,13 @@ struct GatherArgs {
     MaterialView mt;
     const float4* y;   // inertia target (VBD)
     float4* p_new;     // Jacobi output (VBD v4)
+    // Displacement gradients Ju(e, q) = sum_k u_k (x) dN_k(q), 9 floats per
+    // (element, quadrature point), or null to rebuild them from the nodes.
+    // When set (in-place VBD only) the gather reads Ju instead of the S node
+    // positions and, after its vertex moved by du, adds du (x) dN_r(q) to the
+    // records of its incident elements.  Within one colour no two vertices
+    // share an element, so every record has a single writer.
+    float* ju;
     float damping;
     float dt;
     int first_slot;
@@ -35,6 +43,7 @@ struct SceneDev {
     OwnersView own;
     MaterialView mt;
     float4* y;
+    float* ju;   // cached displacement gradients, or null
     float damping;
     int iterations;
     float g[3];
@@ -66,25 +75,57 @@ struct BatchGatherArgs {
     float dt;
 };
 
-bool launch_gather_4_1(int L, int QL, const GatherArgs& a, hipStream_t s);
-bool launch_gather_8_8(int L, int QL, const GatherArgs& a, hipStream_t s);
-bool launch_gather_10_4(int L, int QL, const GatherArgs& a, hipStream_t s);
-bool launch_gather_20_11(int L, int QL, const GatherArgs& a, hipStream_t s);
-bool launch_gather_27_27(int L, int QL, const GatherArgs& a, hipStream_t s);
+// Batched refresh of the cached displacement gradients: VSeg.count is the
+// scene's E * Q records.
+struct BatchRefreshArgs {
+    const SceneDev* scenes;
+    const VSeg* segs;
+    const int* block_seg;
+    int block_base;
+    int iteration;
+};
 
-bool launch_batch_gather_4_1(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
-bool launch_batch_gather_8_8(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
-bool launch_batch_gather_10_4(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
-bool launch_batch_gather_20_11(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
-bool launch_batch_gather_27_27(int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s);
+#define OE_GATHER_DECL(S, Q)                                                                                  \
+    bool launch_gather_##S##_##Q(int L, int QL, const GatherArgs& a, hipStream_t s);                          \
+    bool launch_batch_gather_##S##_##Q(int L, int QL, bool cached, const BatchGatherArgs& b, int nblocks,     \
+                                       hipStream_t s);                                                        \
+    void launch_ju_refresh_##S##_##Q(const PSView& ps, const FemView& f, float* ju, hipStream_t s);          \
+    void launch_batch_ju_refresh_##S##_##Q(const BatchRefreshArgs& b, int nblocks, hipStream_t s);
+OE_GATHER_DECL(4, 1)
+OE_GATHER_DECL(8, 8)
+OE_GATHER_DECL(10, 4)
+OE_GATHER_DECL(20, 11)
+OE_GATHER_DECL(27, 27)
+#undef OE_GATHER_DECL
 
-inline bool launch_batch_gather(int S, int Q, int L, int QL, const BatchGatherArgs& b, int nblocks, hipStream_t s) {
-    if (S == 4 && Q == 1) return launch_batch_gather_4_1(L, QL, b, nblocks, s);
-    if (S == 8 && Q == 8) return launch_batch_gather_8_8(L, QL, b, nblocks, s);
-    if (S == 10 && Q == 4) return launch_batch_gather_10_4(L, QL, b, nblocks, s);
-    if (S == 20 && Q == 11) return launch_batch_gather_20_11(L, QL, b, nblocks, s);
-    if (S == 27 && Q == 27) return launch_batch_gather_27_27(L, QL, b, nblocks, s);
+inline bool launch_batch_gather(int S, int Q, int L, int QL, bool cached, const BatchGatherArgs& b, int nblocks,
+                                hipStream_t s) {
+    if (S == 4 && Q == 1) return launch_batch_gather_4_1(L, QL, cached, b, nblocks, s);
+    if (S == 8 && Q == 8) return launch_batch_gather_8_8(L, QL, cached, b, nblocks, s);
+    if (S == 10 && Q == 4) return launch_batch_gather_10_4(L, QL, cached, b, nblocks, s);
+    if (S == 20 && Q == 11) return launch_batch_gather_20_11(L, QL, cached, b, nblocks, s);
+    if (S == 27 && Q == 27) return launch_batch_gather_27_27(L, QL, cached, b, nblocks, s);
     return false;
+}
+
+inline bool launch_ju_refresh(int S, int Q, const PSView& ps, const FemView& f, float* ju, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_ju_refresh_4_1(ps, f, ju, s);
+    else if (S == 8 && Q == 8) launch_ju_refresh_8_8(ps, f, ju, s);
+    else if (S == 10 && Q == 4) launch_ju_refresh_10_4(ps, f, ju, s);
+    else if (S == 20 && Q == 11) launch_ju_refresh_20_11(ps, f, ju, s);
+    else if (S == 27 && Q == 27) launch_ju_refresh_27_27(ps, f, ju, s);
+    else return false;
+    return true;
+}
+
+inline bool launch_batch_ju_refresh(int S, int Q, const BatchRefreshArgs& b, int nblocks, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_batch_ju_refresh_4_1(b, nblocks, s);
+    else if (S == 8 && Q == 8) launch_batch_ju_refresh_8_8(b, nblocks, s);
+    else if (S == 10 && Q == 4) launch_batch_ju_refresh_10_4(b, nblocks, s);
+    else if (S == 20 && Q == 11) launch_batch_ju_refresh_20_11(b, nblocks, s);
+    else if (S == 27 && Q == 27) launch_batch_ju_refresh_27_27(b, nblocks, s);
+    else return false;
+    return true;
 }
 
 // Dispatch on (S, Q); returns false for an unsupported tile shape.

@@ -203,21 +203,27 @@ GPU_FEM::GPU_FEM(Engine* e, Element el, const HostMesh& mesh, float young, float
     lame_parameters(young, poisson, &mt.lambda, &mt.mu);
     mt.alpha = (material == Stable_NeoHooke) ? 1.f + mt.mu / mt.lambda : 1.f;
     topo = DBuf<int>(&eng->pool, std::vector<int>(mesh.topo.begin(), mesh.topo.end()));
-    jxv = DBuf<float>(&eng->pool, fc.jxv);
+    std::vector<float> rec, w;
+    affine = build_affine_records(mesh, rec, w);
+    if (!affine) w.assign(Q, 1.f);   // V is stored whole in the per-point records
+    jxv = DBuf<float>(&eng->pool, affine ? rec : fc.jxv);
+    wq = DBuf<float>(&eng->pool, w);
     dN = DBuf<float>(&eng->pool, fc.dN);
     elem_volume = std::move(fc.elem_volume);
 }
 
-FemView GPU_FEM::fem_view() const { return FemView{E, S, Q, topo.get(), jxv.get(), dN.get()}; }
+FemView GPU_FEM::fem_view() const {
+    return FemView{E, S, Q, topo.get(), jxv.get(), dN.get(), affine ? 1 : 0, wq.get()};
+}
 
 GatherTile default_tile(Element e) {
     switch (e) {
-        case Tetra: return {8, 1};
-        case Hexa: return {16, 1};
-        case Tetra10: return {16, 2};
-        case Tetra20: return {64, 1};
-        case Hexa27: return {64, 1};
-        default: return {16, 1};
+        case Tetra: return {8, 1, false};
+        case Hexa: return {16, 1, false};
+        case Tetra10: return {16, 2, false};
+        case Tetra20: return {64, 1, false};
+        case Hexa27: return {64, 1, true};
+        default: return {16, 1, false};
     }
 }
 
@@ -252,6 +258,11 @@ GPU_VBD_FEM::~GPU_VBD_FEM() {
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
 }
 
+float* GPU_VBD_FEM::ensure_ju() {
+    if (!ju.get()) ju.allocate(&eng->pool, static_cast<size_t>(E) * Q * 9);
+    return ju.get();
+}
+
 void GPU_VBD_FEM::set_shuffle_seed(uint64_t seed) {
     shuffle = seed != 0;
     rng.seed(seed);
@@ -275,6 +286,12 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
     const bool jacobi = (version == Better_Coloration);
     a.mode = jacobi ? GATHER_VBD_JACOBI : GATHER_VBD_INPLACE;
     hipStream_t s = eng->stream;
+    a.ju = nullptr;
+    if (cached()) {
+        // gradients of this iteration's starting positions
+        a.ju = ensure_ju();
+        if (!launch_ju_refresh(S, Q, a.ps, a.fem, a.ju, s)) throw std::invalid_argument("unsupported element");
+    }
     for (int c : order) {
         a.first_slot = owners.color_begin[c];
         a.count = owners.color_begin[c + 1] - a.first_slot;
@@ -356,6 +373,7 @@ void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
     a.mt = mt;
     a.damping = _damping;
     a.dt = dt;
+    a.ju = nullptr;
     a.first_slot = 0;
     a.count = n_vertices;
     a.mode = GATHER_EXPLICIT_FORCE;

@@ -114,17 +114,21 @@ struct GPU_FEM : GPU_Dynamic {
     Element element;
     int S, Q, E;
     MaterialView mt;
+    bool affine = false;   // element records stored once per element
     DBuf<int> topo;
-    DBuf<float> jxv, dN;
+    DBuf<float> jxv, dN, wq;
     std::vector<double> elem_volume;
 };
 
 enum VBD_Version : int { Base = 0, Threaded_Quadrature = 1, Reduction_Symmetry = 2, Better_Coloration = 3 };
 
-// Tile shape of the per-vertex gather: lanes per vertex, quadrature points per lane.
+// Tile shape of the per-vertex gather: lanes per vertex, quadrature points per
+// lane, and whether the VBD sweep keeps the displacement gradients per
+// (element, quadrature point) between colours (QL = 1 only).
 struct GatherTile {
     int L = 16;
     int QL = 1;
+    bool cache = false;
 };
 GatherTile default_tile(Element e);
 
@@ -149,6 +153,9 @@ struct GPU_VBD_FEM final : GPU_FEM {
     DBuf<int2> d_slot_nbo;
     DBuf<int> d_own;
     DBuf<float4> p_new;
+    DBuf<float> ju;           // cached displacement gradients, E * Q * 9 (allocated on first use)
+    float* ensure_ju();
+    bool cached() const { return tile.cache && version != Better_Coloration; }
     int64_t conflicts = 0;
     // optional per-launch timing of the gather kernel (events around each colour)
     bool time_gather = false;

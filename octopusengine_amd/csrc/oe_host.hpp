@@ -107,6 +107,13 @@ struct FemConstants {
 FemConstants build_fem_constants(const HostMesh& m);
 std::vector<double> element_volumes(const HostMesh& m);
 
+// Per-element records for meshes whose every element is an affine image of
+// the reference element (X_k = a + B xi_k for all nodes, to within 8 float
+// ulps of the coordinates): rec = E * 10 floats (B^-1 column-major, |det B|)
+// and wq = Q quadrature weights, so that V_q = |det B| wq[q].  Returns false
+// (outputs untouched) when some element is not affine.
+bool build_affine_records(const HostMesh& m, std::vector<float>& rec, std::vector<float>& wq);
+
 // Reference src/Dynamic/FEM/FEM_Shape.cpp:86-121.
 std::vector<float> compute_fem_mass(const HostMesh& m, const FemConstants& fc, float density,
                                     Mass_Distribution distrib);

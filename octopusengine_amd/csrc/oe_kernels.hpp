@@ -13,6 +13,18 @@
 
 namespace oe {
 
+// Block order for the 8 XCDs of the chip: the dispatcher hands work-group i to
+// XCD i % 8, so consecutive block ids land in different L2 caches.  This maps
+// the hardware block id to a logical one such that each XCD walks one
+// contiguous range of logical blocks (a bijection on [0, n)); neighbouring
+// vertices of a scene, which share elements and nodes, then share an L2.
+__device__ __forceinline__ int xcd_block(int bid, int n) {
+    constexpr int X = 8;
+    const int q = n / X, r = n - (n / X) * X;
+    const int xcd = bid % X, idx = bid / X;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
 // Per-vertex accumulator: force (3) + lower triangle of the symmetric 3x3
 // Hessian block (H00, H10, H20, H11, H21, H22).
 struct Acc9 {
@@ -112,12 +124,8 @@ __device__ __forceinline__ float det_from_cof(const float F[9], const float C[9]
 // The reference assembles H_r from six 3x3 blocks of d2W/dF2; the
 // lambda (J - alpha) d2J/dF2 blocks are antisymmetric and cancel in the
 // symmetric assembly, which leaves the closed form above.
-__device__ __forceinline__ void nh_contrib(const float Jx[9], const float* __restrict__ rec, const float* __restrict__ dnr,
+__device__ __forceinline__ void nh_contrib(const float Jx[9], const float JXi[9], float V, const float* __restrict__ dnr,
                                            const MaterialView& mt, Acc9& a) {
-    float JXi[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
-    const float V = rec[9];
     float F[9];
     deformation_gradient(Jx, JXi, F);
     // g_c = sum_l JX_inv(l, c) dN_r[l]

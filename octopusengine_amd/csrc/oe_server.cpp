@@ -39,7 +39,7 @@ int Server::add_scene(Scene* s) {
     // the device table is reallocated only while nothing is in flight
     idle_cv_.wait(lk, [&] { return inflight_.empty() || !error_.empty(); });
     flags_.emplace_back(&eng_->pool, fixed_flags(*s));
-    host_table_.push_back(make_scene_dev(*s, flags_.back().get()));
+    host_table_.push_back(make_scene_dev(*s, flags_.back().get(), tiles_));
     scenes_.push_back(s);
     scene_busy_.push_back(0);
     grow_table();

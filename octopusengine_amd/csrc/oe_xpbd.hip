@@ -86,15 +86,13 @@ __global__ __launch_bounds__(256) void k_xpbd_v0(XpbdArgs a) {
         float C = 0.f;
         for (int q = 0; q < Q; ++q) {
             const float* dn = a.fem.dN + q * S * 3;
-            const float* rec = a.fem.jxv + (static_cast<size_t>(eid) * Q + q) * 10;
-            float Jx[9], JXi[9], F[9], P[9], Ps[9], e;
+            float Jx[9], JXi[9], F[9], P[9], Ps[9], e, V;
+            load_element_record(a.fem, eid, q, JXi, V);
             jx_from_nodes<S>(xs, dn, Jx);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
             deformation_gradient(Jx, JXi, F);
             xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
-            scale_stress(P, JXi, rec[9], Ps);
-            C += e * rec[9];
+            scale_stress(P, JXi, V, Ps);
+            C += e * V;
 #pragma unroll
             for (int k = 0; k < S; ++k)
 #pragma unroll
@@ -153,8 +151,8 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
         if (valid && lane < Q) {
             const int q = lane;
             const float* dn = a.fem.dN + q * S * 3;
-            const float* rec = a.fem.jxv + (static_cast<size_t>(eid) * Q + q) * 10;
-            float Jx[9], JXi[9], F[9], P[9], Ps[9], e;
+            float Jx[9], JXi[9], F[9], P[9], Ps[9], e, V;
+            load_element_record(a.fem, eid, q, JXi, V);
 #pragma unroll
             for (int i = 0; i < 9; ++i) Jx[i] = 0.f;
             for (int k = 0; k < S; ++k) {
@@ -165,12 +163,10 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
                     Jx[r] += xr * d0; Jx[3 + r] += xr * d1; Jx[6 + r] += xr * d2;
                 }
             }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) JXi[i] = rec[i];
             deformation_gradient(Jx, JXi, F);
             xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
-            scale_stress(P, JXi, rec[9], Ps);
-            e_loc = e * rec[9];
+            scale_stress(P, JXi, V, Ps);
+            e_loc = e * V;
 #pragma unroll
             for (int i = 0; i < 9; ++i) sP[le][q][i] = Ps[i];
         }

@@ -16,13 +16,24 @@ def fixed_ids(xyz):
     return np.nonzero(xyz[:, 0] < 1e-6)[0].astype(np.int32)
 
 
+def curve_rest(xyz, amplitude):
+    """Smooth non-affine map of the rest shape that keeps x (and so the fixed
+    x = 0 face): higher-order nodes leave the straight edges and the elements
+    become curved."""
+    x = xyz[:, 0].astype(np.float64)
+    out = xyz.astype(np.float64).copy()
+    out[:, 1] += amplitude * np.sin(3.0 * x) * (1.0 + 4.0 * xyz[:, 2])
+    out[:, 2] += amplitude * np.cos(2.0 * x) * xyz[:, 1] * 4.0
+    return out.astype(np.float32)
+
+
 def make_pair(engine, element, cells, size=(1.0, 0.25, 0.25), *, solver=0, material=3, young=5e5, poisson=0.49,
               density=1000.0, iterations=10, damping=0.0, global_damping=0.0, xpbd_version=2, femshape=True,
-              noise=0.0, seed=0, box=True, fix=True, tile=None):
+              noise=0.0, seed=0, box=True, fix=True, tile=None, curve=0.0):
     import octopusengine_amd as oe
 
     xyz, topo = oe.beam_mesh(element, cells, size)
-    rest = xyz.copy()
+    rest = curve_rest(xyz, curve) if curve else xyz.copy()
     g = oe.Scene(engine, element, rest, topo, material=material, young=young, poisson=poisson, density=density,
                  distribution=oe.MASS_FEMSHAPE if femshape else oe.MASS_UNIFORM, solver=solver,
                  iterations=iterations, damping=damping, global_damping=global_damping, xpbd_version=xpbd_version)

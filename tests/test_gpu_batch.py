@@ -49,6 +49,27 @@ def test_batch_equals_alone_and_oracle(engine):
     b.close()
 
 
+def test_batch_tiles_with_gradient_cache(engine):
+    """Per-element-type tiles, with and without the gradient cache, give each
+    scene the step it gets alone with the same tile."""
+    tiles = {oe.TETRA10: (16, 1, True), oe.HEXA27: (32, 1, False), oe.TETRA: (8, 1, True),
+             oe.TETRA20: (64, 1, True), oe.HEXA: (16, 1, True)}
+    batched = [_make(engine, s)[0] for s in MIX]
+    alone = [_make(engine, s, tile=tiles[s["el"]])[0] for s in MIX]
+    b = oe.Batch(engine, batched)
+    for el, t in tiles.items():
+        b.set_tile(el, *t)
+    for _ in range(2):
+        b.step(0.005, 1)
+        for a in alone:
+            a.step(0.005, 1)
+    for g, a, spec in zip(batched, alone, MIX):
+        err = max_err(g.positions(), a.positions())
+        print(f"tiles el={spec['el']} |batch-alone|={err:.3e}", flush=True)
+        assert err <= 1e-6
+    b.close()
+
+
 def test_batch_graph_replay_matches_eager(engine):
     s1 = [_make(engine, s)[0] for s in MIX[:4]]
     s2 = [_make(engine, s)[0] for s in MIX[:4]]

@@ -84,6 +84,80 @@ def test_vbd_tiles_agree(engine):
         assert err <= TOL1
 
 
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+def test_vbd_cached_gradients(engine, case):
+    """The sweep that keeps Ju(e, q) between colours and updates it with each
+    vertex move computes the step of the sweep that rebuilds it from the
+    nodes, and both match the oracle."""
+    el, cells = case
+    lanes = 16 if el in (3, 7) else 64
+    ref = oref = None
+    for cache in (False, True):
+        g, o, _, _ = make_pair(engine, el, cells, tile=(lanes, 1, cache))
+        assert g.get_tile() == (lanes, 1, cache)
+        for _ in range(2):
+            g.step(0.005, 1)
+        p = g.positions()
+        if ref is None:
+            for _ in range(2):
+                o.step(0.005, 1)
+            oref = o.positions()
+            e_orc = max_err(p, oref)
+            ref = p
+        else:
+            e_orc = max_err(p, oref)
+            e_pair = max_err(p, ref)
+            print(f"el={el} cached vs rebuilt {e_pair:.3e}", flush=True)
+            assert e_pair <= TOL1 / 10
+        print(f"el={el} cache={cache} 2-step err={e_orc:.3e}", flush=True)
+        assert e_orc <= TOL1
+
+
+def test_gradient_cache_tile_checks(engine):
+    import octopusengine_amd as oe
+
+    g, _, _, _ = make_pair(engine, 7, (2, 1, 1))
+    with pytest.raises(oe.OEError):
+        g.set_tile(16, 2, True)          # the cache takes one quadrature point per lane
+    x, _, _, _ = make_pair(engine, 7, (2, 1, 1), solver=oe.SOLVER_EXPLICIT, iterations=1)
+    with pytest.raises(oe.OEError):
+        x.set_tile(16, 1, True)          # VBD option only
+    x.set_tile(16, 1, False)
+
+
+def test_vbd_cached_gradients_many_steps(engine):
+    g, o, rest, _ = make_pair(engine, 9, (4, 2, 2), tile=(64, 1, True))
+    for _ in range(40):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    err = max_err(g.positions(), o.positions())
+    disp = float(np.abs(o.positions() - rest).max())
+    print(f"Q2 cached 40 steps err={err:.3e} displacement={disp:.3e}", flush=True)
+    assert disp > 1e-3
+    assert err <= TOL100
+
+
+CURVED = [(7, (6, 2, 2)), (8, (3, 1, 1)), (9, (4, 2, 2)), (6, (6, 2, 2))]
+
+
+@pytest.mark.parametrize("case", CURVED, ids=[_ids(c) for c in CURVED])
+def test_vbd_curved_elements(engine, case):
+    """Curved elements keep one record per (element, quadrature point); the
+    straight beam of the same element type uses per-element records."""
+    el, cells = case
+    straight, _, _, _ = make_pair(engine, el, cells)
+    assert straight.affine()
+    for cache in (False, True):
+        lanes = 16 if el == 7 else 64
+        g, o, _, _ = make_pair(engine, el, cells, curve=0.02, tile=(lanes, 1, cache))
+        assert not g.affine()
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+        err = max_err(g.positions(), o.positions())
+        print(f"curved el={el} cache={cache} 1-step err={err:.3e}", flush=True)
+        assert err <= TOL1
+
+
 def test_vbd_graph_replay_matches_eager(engine):
     a, _, _, _ = make_pair(engine, 9, (4, 2, 2))
     b, _, _, _ = make_pair(engine, 9, (4, 2, 2))

@@ -101,6 +101,43 @@ def test_fem_constants_and_masses(el, cells):
         assert np.abs(m1 - m2).max() <= 1e-6 * m2.max()
 
 
+AFFINE_MAP = np.array([[1.0, 0.2, -0.1], [0.05, 0.9, 0.3], [-0.2, 0.1, 1.1]])
+
+
+@pytest.mark.parametrize("sheared", [False, True])
+@pytest.mark.parametrize("el,cells", CASES)
+def test_affine_records_match_per_point_constants(el, cells, sheared):
+    """On straight-sided meshes (the beam, and the beam under a general affine
+    map) the per-element record reproduces the oracle's per-(element,
+    quadrature point) JX_inv and V = |det J_q| w_q at every point."""
+    xyz, topo = oe.beam_mesh(el, cells)
+    if sheared:
+        xyz = (xyz.astype(np.float64) @ AFFINE_MAP.T + [0.3, -0.2, 0.1]).astype(np.float32)
+    out = oe.host_affine_records(el, xyz, topo)
+    assert out is not None
+    rec, w = out
+    ji, V = O.fem_constants(el, xyz, topo)
+    _, wo = O.quadrature(el)
+    assert np.array_equal(w, np.asarray(wo, np.float32))
+    assert np.abs(rec[:, None, :9] - ji).max() <= 1e-5 * np.abs(ji).max()
+    assert np.abs(rec[:, None, 9] * w[None, :] - V).max() <= 1e-5 * V.max()
+
+
+@pytest.mark.parametrize("el,cells,node", [(oe.TETRA10, (4, 2, 2), 4), (oe.TETRA20, (3, 2, 2), 7),
+                                           (oe.HEXA, (4, 2, 2), 0), (oe.HEXA27, (3, 2, 2), 13)])
+def test_affine_records_reject_curved_elements(el, cells, node):
+    """Moving one node of one element off the affine image (by 1e-3 of the
+    cell size) makes the mesh curved: no per-element records."""
+    xyz, topo = oe.beam_mesh(el, cells)
+    xyz = xyz.copy()
+    xyz[topo[1, node], 1] += 1e-3 * 0.25 / cells[1]
+    assert oe.host_affine_records(el, xyz, topo) is None
+    # P1 elements are always affine
+    x1, t1 = oe.beam_mesh(oe.TETRA, (4, 2, 2))
+    x1 = x1 + np.random.default_rng(0).uniform(-1e-3, 1e-3, x1.shape).astype(np.float32)
+    assert oe.host_affine_records(oe.TETRA, x1, t1) is not None
+
+
 def test_invalid_inputs_return_status():
     bad = np.array([[0, 1, 2, 99]], np.int32)
     xyz = np.zeros((4, 3), np.float32)
