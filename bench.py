@@ -102,6 +102,25 @@ def timed(fn, steps, sync):
     return time.perf_counter() - t0
 
 
+def measured_traffic(kernel_prefix: str, workload: str):
+    """HBM bytes per launch of a kernel on a workload, from the rocprof
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json (written
+    from the counter CSVs; each counter already scaled by its copy-kernel
+    calibration).  None when no such measurement exists."""
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        rec = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("kernel", "").startswith(kernel_prefix) and r.get("workload") == workload:
+            return {"bytes_per_launch": r["read_bytes_per_launch"] + r["write_bytes_per_launch"],
+                    "read_bytes_per_launch": r["read_bytes_per_launch"],
+                    "write_bytes_per_launch": r["write_bytes_per_launch"],
+                    "dispatches": r["dispatches"], "source": f"profiles/traffic.json ({rec.get('commit', '?')})"}
+    return None
+
+
 def batched_leg(args, eng) -> dict:
     import octopusengine_amd as oe
 
@@ -148,6 +167,7 @@ def batched_leg(args, eng) -> dict:
     bytes_total = info["gather_bytes"] * args.steps
     avg_launch_s = gather_ms / 1e3 / max(launches, 1)
     achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
+    traffic = measured_traffic("k_batch_gather", f"batched{args.scenes}_seed{args.seed}")
     b.close()
     return {
         "value": vu_step * args.steps / wall,
@@ -162,7 +182,9 @@ def batched_leg(args, eng) -> dict:
         "scenes": len(scenes), "p2_scenes": n_p2, "vertices": nv, "vertex_updates_per_step": vu_step,
         "gather_launches_per_step": info["gather_launches"], "setup_s": setup_s, "positions_finite": finite,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_detail": traffic,
                      "kernel": "k_batch_gather (VBD colour pass, all scenes)",
                      "bytes_per_launch": bytes_total / max(launches, 1), "avg_launch_us": avg_launch_s * 1e6,
                      "launches": launches},

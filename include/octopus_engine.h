@@ -206,6 +206,25 @@ oe_status oe_host_affine_records(int32_t element, const float* xyz, int32_t n_ve
 oe_status oe_host_masses(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
                          int32_t n_elements, float density, int32_t distribution, float* masses);
 
+/* ------------------------------------------------------------ VTK output */
+/* Legacy-VTK 5.1 ASCII unstructured grid in the layout of the reference's
+ * VTK_Formater (src/Mesh/Converter/VTK_Formater.cpp): open, mesh (POINTS,
+ * CELLS/OFFSETS/CONNECTIVITY, CELL_TYPES; Hexa27 cells in VTK node order),
+ * then point_data / cell_data blocks of scalars (n floats) and vectors (3n
+ * floats), close (flushes and frees the handle). */
+typedef struct oe_vtk oe_vtk;
+oe_status oe_vtk_open(const char* path, oe_vtk** out);
+oe_status oe_vtk_mesh(oe_vtk* v, int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
+                      int32_t n_elements);
+oe_status oe_vtk_point_data(oe_vtk* v);
+oe_status oe_vtk_cell_data(oe_vtk* v);
+oe_status oe_vtk_scalars(oe_vtk* v, const char* name, const float* data, int32_t n);
+oe_status oe_vtk_vectors(oe_vtk* v, const char* name, const float* data, int32_t n);
+oe_status oe_vtk_close(oe_vtk* v);
+/* Scene snapshot as the reference's result files hold it: the rest geometry,
+ * point data "weights" (vertex masses) and "u" (displacement). Synchronising. */
+oe_status oe_scene_save_vtk(oe_scene* s, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

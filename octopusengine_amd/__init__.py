@@ -114,6 +114,45 @@ def host_masses(element: int, xyz, topo, density: float, distribution: int):
     return out
 
 
+class VtkWriter:
+    """Legacy-VTK 5.1 ASCII writer with the reference's file layout
+    (VTK_Formater: open, save_mesh, start_point_data / start_cell_data,
+    add_scalar_data / add_vector_data, close)."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        L.check(L.lib.oe_vtk_open(str(path).encode(), C.byref(h)))
+        self._h = h
+
+    def mesh(self, element: int, xyz, topo):
+        xyz, topo = _f32(xyz), _i32(topo)
+        L.check(L.lib.oe_vtk_mesh(self._h, element, L.fptr(xyz), xyz.shape[0], L.iptr(topo), topo.shape[0]))
+        return self
+
+    def point_data(self):
+        L.check(L.lib.oe_vtk_point_data(self._h))
+        return self
+
+    def cell_data(self):
+        L.check(L.lib.oe_vtk_cell_data(self._h))
+        return self
+
+    def scalars(self, name: str, data):
+        d = _f32(data).reshape(-1)
+        L.check(L.lib.oe_vtk_scalars(self._h, name.encode(), L.fptr(d), d.size))
+        return self
+
+    def vectors(self, name: str, data):
+        d = _f32(data).reshape(-1, 3)
+        L.check(L.lib.oe_vtk_vectors(self._h, name.encode(), L.fptr(d), d.shape[0]))
+        return self
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            L.check(L.lib.oe_vtk_close(h))
+
+
 # ------------------------------------------------------------------ device side
 class Engine:
     """One GPU, one stream, one device-memory budget (bytes, 0 = none)."""
@@ -220,6 +259,10 @@ class Scene:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         L.check(L.lib.oe_scene_get_tile(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, bool(c.value)
+
+    def save_vtk(self, path):
+        """Rest geometry + point data "weights" (masses) and "u" (displacement)."""
+        L.check(L.lib.oe_scene_save_vtk(self._h, str(path).encode()))
 
     def affine(self) -> bool:
         """True when the element records are stored once per element."""

@@ -121,6 +121,14 @@ _SIGS = {
     "oe_host_color_dual": (C.c_int32, [C.c_int32, i32p, C.c_int32, C.c_int32, i32p, i32p, i64p]),
     "oe_host_fem_constants": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, f32p]),
     "oe_host_masses": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, C.c_float, C.c_int32, f32p]),
+    "oe_vtk_open": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "oe_vtk_mesh": (C.c_int32, [P, C.c_int32, f32p, C.c_int32, i32p, C.c_int32]),
+    "oe_vtk_point_data": (C.c_int32, [P]),
+    "oe_vtk_cell_data": (C.c_int32, [P]),
+    "oe_vtk_scalars": (C.c_int32, [P, C.c_char_p, f32p, C.c_int32]),
+    "oe_vtk_vectors": (C.c_int32, [P, C.c_char_p, f32p, C.c_int32]),
+    "oe_vtk_close": (C.c_int32, [P]),
+    "oe_scene_save_vtk": (C.c_int32, [P, C.c_char_p]),
     "oe_host_affine_records": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, f32p, f32p, i32p]),
 }
 

@@ -26,6 +26,9 @@ struct oe_server {
     std::unique_ptr<oe::Server> server;
     oe_engine* owner;
 };
+struct oe_vtk {
+    std::unique_ptr<oe::VtkWriter> w;
+};
 
 namespace {
 
@@ -584,6 +587,85 @@ oe_status oe_host_masses(int32_t element, const float* xyz, int32_t nv, const in
         const std::vector<float> ms =
             oe::compute_fem_mass(m, fc, density, static_cast<oe::Mass_Distribution>(distribution));
         if (masses) std::memcpy(masses, ms.data(), ms.size() * sizeof(float));
+    });
+}
+
+// ------------------------------------------------------------------ VTK I/O
+oe_status oe_vtk_open(const char* path, oe_vtk** out) {
+    return guard([&] {
+        OE_REQUIRE(path && out, "bad argument");
+        auto* v = new oe_vtk();
+        try {
+            v->w = std::make_unique<oe::VtkWriter>(path);
+        } catch (...) {
+            delete v;
+            throw;
+        }
+        *out = v;
+    });
+}
+
+oe_status oe_vtk_mesh(oe_vtk* v, int32_t element, const float* xyz, int32_t nv, const int32_t* topology, int32_t ne) {
+    return guard([&] {
+        OE_REQUIRE(v && v->w && oe::element_supported(element), "bad argument");
+        OE_REQUIRE((xyz || nv == 0) && (topology || ne == 0), "bad argument");
+        v->w->mesh(static_cast<oe::Element>(element), xyz, nv, topology, ne);
+    });
+}
+
+oe_status oe_vtk_point_data(oe_vtk* v) {
+    return guard([&] {
+        OE_REQUIRE(v && v->w, "bad argument");
+        v->w->point_data();
+    });
+}
+
+oe_status oe_vtk_cell_data(oe_vtk* v) {
+    return guard([&] {
+        OE_REQUIRE(v && v->w, "bad argument");
+        v->w->cell_data();
+    });
+}
+
+oe_status oe_vtk_scalars(oe_vtk* v, const char* name, const float* data, int32_t n) {
+    return guard([&] {
+        OE_REQUIRE(v && v->w && name && (data || n == 0), "bad argument");
+        v->w->scalars(name, data, n);
+    });
+}
+
+oe_status oe_vtk_vectors(oe_vtk* v, const char* name, const float* data, int32_t n) {
+    return guard([&] {
+        OE_REQUIRE(v && v->w && name && (data || n == 0), "bad argument");
+        v->w->vectors(name, data, n);
+    });
+}
+
+oe_status oe_vtk_close(oe_vtk* v) {
+    return guard([&] {
+        OE_REQUIRE(v, "bad argument");
+        std::unique_ptr<oe_vtk> own(v);
+        if (own->w) own->w->close();
+    });
+}
+
+oe_status oe_scene_save_vtk(oe_scene* s, const char* path) {
+    return guard([&] {
+        OE_REQUIRE(s && path, "bad argument");
+        oe::Scene& sc = *s->scene;
+        std::vector<float> rest, pos, m;
+        sc.ps->get_init_position(rest);
+        sc.ps->get_position(pos);
+        sc.ps->get_mass(m);
+        std::vector<float> u(pos.size());
+        for (size_t i = 0; i < u.size(); ++i) u[i] = pos[i] - rest[i];
+        const int nv = sc.mesh.nb_vertices();
+        oe::VtkWriter w(path);
+        w.mesh(sc.mesh.element, rest.data(), nv, sc.mesh.topo.data(), sc.mesh.nb_elements());
+        w.point_data();
+        w.scalars("weights", m.data(), nv);
+        w.vectors("u", u.data(), nv);
+        w.close();
     });
 }
 

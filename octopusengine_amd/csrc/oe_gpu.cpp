@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 
@@ -165,7 +166,8 @@ void GPU_VBD::add_dynamics(GPU_Dynamic* d) {
 
 void GPU_VBD::step(GPU_ParticleSystem* ps, float dt) {
     const PSView v = ps->get_parameters();
-    const bool cheb = _rho > 1.1920929e-7f;
+    // rho > eps with the reference's eps = FLT_MIN * 1e6 (include/Core/Base.h:11)
+    const bool cheb = _rho > std::numeric_limits<float>::min() * 1e6f;
     init_dynamics(ps, dt);
     launch_vbd_predict(v, y.get(), cheb ? prev_it_p.get() : nullptr, dt, gravity, stream);
     float omega = 1.f;

@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cstdint>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -142,5 +143,27 @@ struct ElementSchedule {
     std::vector<int32_t> color_begin;   // nb_color + 1
 };
 ElementSchedule build_element_schedule(const Coloration& col, int nb_elements);
+
+// Legacy-VTK 5.1 ASCII unstructured-grid writer with the reference's layout
+// (VTK_Formater, src/Mesh/Converter/VTK_Formater.cpp): mesh(), then
+// point_data() / cell_data() blocks of scalars() and vectors().
+int vtk_cell_type(Element e);
+int vtk_hexa27_node(int vtk_position);
+class VtkWriter {
+public:
+    explicit VtkWriter(const std::string& path);
+    void mesh(Element e, const float* xyz, int nv, const int32_t* topo, int ne);
+    void point_data();
+    void cell_data();
+    void scalars(const std::string& name, const float* data, int n);
+    void vectors(const std::string& name, const float* data, int n);   // 3n floats
+    void close();
+
+private:
+    void check_size(int n) const;
+    std::ofstream out_;
+    int nb_vertices_ = 0, nb_cells_ = 0;
+    int mode_ = 0;   // 1 point data, 2 cell data
+};
 
 }  // namespace oe

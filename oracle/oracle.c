@@ -994,6 +994,10 @@ struct orc_scene {
     double com[3];
     int has_box;
     double pmin[3], pmax[3];
+    /* VBD options: Chebyshev radius, Jacobi colour sweep (v4) */
+    double rho;
+    int jacobi;
+    double *prev1, *prev2, *pnew;
 };
 
 orc_scene* orc_scene_create(int element, int nv, const float* xyz, int ne, const int* topo, int material, double young,
@@ -1076,7 +1080,31 @@ void orc_scene_destroy(orc_scene* s) {
     free(s->p); free(s->last_p); free(s->init_p); free(s->v); free(s->f); free(s->y);
     free(s->m); free(s->w); free(s->mask); free(s->colors); free(s->order);
     free(s->inc_off); free(s->inc); free(s->fix);
+    free(s->prev1); free(s->prev2); free(s->pnew);
     free(s);
+}
+
+/* GPU_VBD(ps, it, rho) and GPU_VBD_FEM(..., version, target_color)
+ * (include/GPU/VBD/GPU_VBD.h:15, include/GPU/VBD/GPU_VBD_FEM.h:62-65):
+ * rho > 0 turns on the Chebyshev acceleration; jacobi = 1 is the v4 sweep
+ * (kernel_vbd_solve_v4 + kernel_copy_in_position, GPU_VBD_FEM.cu:345-413);
+ * target_color > 0 recolours with ColorReduction (Graph.h:941-969). */
+int orc_scene_vbd_options(orc_scene* s, double rho, int jacobi, int target_color) {
+    int i;
+    if (s->solver != ORC_VBD) return -1;
+    s->rho = rho;
+    s->jacobi = jacobi;
+    if (target_color > 0) {
+        s->ncolor = orc_color_dsat(s->element, s->topo, s->E, s->N, target_color, s->colors);
+        free(s->order);
+        s->order = NULL;
+    }
+    free(s->prev1); free(s->prev2); free(s->pnew);
+    s->prev1 = (double*)malloc(sizeof(double) * 3 * (size_t)s->N);
+    s->prev2 = (double*)malloc(sizeof(double) * 3 * (size_t)s->N);
+    s->pnew = (double*)malloc(sizeof(double) * 3 * (size_t)s->N);
+    for (i = 0; i < 3 * s->N; ++i) s->prev1[i] = s->prev2[i] = s->pnew[i] = s->p[i];
+    return s->ncolor;
 }
 
 /* GPU_Fix_Constraint ctor (GPU_Constraint.cu:94-107): com = mean of the fixed
@@ -1172,8 +1200,8 @@ static void vertex_fH(const orc_scene* s, int v, double f[3], double H[9]) {
     }
 }
 
-/* compute_correction (GPU_VBD_FEM.cu:107-122) */
-static void vbd_solve_vertex(orc_scene* s, int v, double dt) {
+/* compute_correction (GPU_VBD_FEM.cu:107-122): writes p + dx to out */
+static void vbd_solve_vertex(orc_scene* s, int v, double dt, double* out) {
     double f[3], H[9], Hi[9], d;
     const double kd = s->damping / dt, mh2 = s->m[v] / (dt * dt);
     int k;
@@ -1189,17 +1217,22 @@ static void vbd_solve_vertex(orc_scene* s, int v, double dt) {
     H[4] += mh2;
     H[8] += mh2;
     d = fabs(det3(H));
+    for (k = 0; k < 3; ++k) out[k] = s->p[3 * v + k];
     if (!(d > 1e-6)) return;
     inv3(H, Hi);
-    for (k = 0; k < 3; ++k) s->p[3 * v + k] += Hi[k] * f[0] + Hi[3 + k] * f[1] + Hi[6 + k] * f[2];
+    for (k = 0; k < 3; ++k) out[k] += Hi[k] * f[0] + Hi[3 + k] * f[1] + Hi[6 + k] * f[2];
 }
 
 /* GPU_VBD::step (GPU_VBD.cu:38-61) with GPU_VBD_FEM::step (GPU_VBD_FEM.cu:712-756)
  * in a fixed colour order; vertices of one colour are independent. */
 static void vbd_substep(orc_scene* s, double dt) {
     int i, k, j, c;
+    const int cheb = s->rho > 1.1754943508222875e-38 * 1e6 && s->prev1; /* rho > eps (Core/Base.h:11) */
+    float omega = 1.f;
     /* kernel_integration (GPU_VBD.cu:6-18) */
     for (i = 0; i < s->N; ++i) {
+        if (cheb)
+            for (k = 0; k < 3; ++k) s->prev1[3 * i + k] = s->p[3 * i + k];
         for (k = 0; k < 3; ++k) {
             const double a = s->g[k] + s->f[3 * i + k] * s->w[i];
             const double y = s->p[3 * i + k] + ((s->p[3 * i + k] - s->last_p[3 * i + k]) / dt + a * dt) * dt;
@@ -1212,12 +1245,34 @@ static void vbd_substep(orc_scene* s, double dt) {
     for (j = 0; j < s->iterations; ++j) {
         for (c = 0; c < s->ncolor; ++c) {
             const int col = s->order ? s->order[c] : c;
-            for (i = 0; i < s->N; ++i) {
-                if (s->colors[i] != col || s->mask[i] == 0) continue;
-                vbd_solve_vertex(s, i, dt);
+            if (s->jacobi) {
+                /* every vertex of the colour from the same positions, then copied */
+                for (i = 0; i < s->N; ++i)
+                    if (s->colors[i] == col && s->mask[i] != 0) vbd_solve_vertex(s, i, dt, s->pnew + 3 * (size_t)i);
+                for (i = 0; i < s->N; ++i)
+                    if (s->colors[i] == col && s->mask[i] != 0)
+                        for (k = 0; k < 3; ++k) s->p[3 * i + k] = s->pnew[3 * i + k];
+            } else {
+                double out[3];
+                for (i = 0; i < s->N; ++i) {
+                    if (s->colors[i] != col || s->mask[i] == 0) continue;
+                    vbd_solve_vertex(s, i, dt, out);
+                    for (k = 0; k < 3; ++k) s->p[3 * i + k] = out[k];
+                }
             }
         }
         apply_constraints(s);
+        /* kernel_chebychev_acceleration (GPU_VBD.cu:27-35), omega in float (:53-55) */
+        if (cheb) {
+            const float r = (float)s->rho;
+            if (j == 1) omega = 2.f / (2.f - r * r);
+            else if (j > 1) omega = 4.f / (4.f - r * r * omega);
+            for (i = 0; i < 3 * s->N; ++i) {
+                if (j >= 2) s->p[i] = s->prev2[i] + (double)omega * (s->p[i] - s->prev2[i]);
+                s->prev2[i] = s->prev1[i];
+                s->prev1[i] = s->p[i];
+            }
+        }
     }
     /* kernel_velocity_update (GPU_VBD.cu:20-25) */
     for (i = 0; i < 3 * s->N; ++i) s->v[i] = (s->p[i] - s->last_p[i]) / dt;

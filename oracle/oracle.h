@@ -61,6 +61,9 @@ orc_scene* orc_scene_create(int element, int nv, const float* xyz, int ne, const
 void orc_scene_destroy(orc_scene* s);
 int orc_scene_fix(orc_scene* s, const int* ids, int n);
 void orc_scene_box(orc_scene* s, const double pmin[3], const double pmax[3]);
+/* VBD options: Chebyshev radius (0 = off), Jacobi colour sweep (v4), colour
+ * reduction target (> 0); returns the colour count */
+int orc_scene_vbd_options(orc_scene* s, double rho, int jacobi, int target_color);
 /* colour-order override (VBD): ncolor entries, NULL = ascending */
 void orc_scene_color_order(orc_scene* s, const int* order);
 void orc_scene_step(orc_scene* s, double dt, int substeps);

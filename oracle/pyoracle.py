@@ -58,6 +58,7 @@ lib.orc_scene_destroy.argtypes = [P]
 lib.orc_scene_fix.argtypes = [P, i32p, C.c_int]
 lib.orc_scene_box.argtypes = [P, f64p, f64p]
 lib.orc_scene_color_order.argtypes = [P, i32p]
+lib.orc_scene_vbd_options.argtypes = [P, C.c_double, C.c_int, C.c_int]
 lib.orc_scene_step.argtypes = [P, C.c_double, C.c_int]
 lib.orc_scene_get.argtypes = [P, C.c_int, f64p]
 lib.orc_scene_set.argtypes = [P, C.c_int, f64p]
@@ -202,6 +203,13 @@ class Scene:
 
     def box(self, pmin=(-10, -5, -10), pmax=(10, 5, 10)):
         lib.orc_scene_box(self._h, dp(_f64(pmin)), dp(_f64(pmax)))
+
+    def vbd_options(self, rho=0.0, jacobi=False, target_color=-1):
+        """Chebyshev radius, Jacobi (v4) colour sweep, colour reduction target."""
+        n = lib.orc_scene_vbd_options(self._h, rho, int(jacobi), target_color)
+        if n < 0:
+            raise ValueError("VBD options on a non-VBD scene")
+        return n
 
     def color_order(self, order):
         if order is None:

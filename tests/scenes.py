@@ -29,17 +29,21 @@ def curve_rest(xyz, amplitude):
 
 def make_pair(engine, element, cells, size=(1.0, 0.25, 0.25), *, solver=0, material=3, young=5e5, poisson=0.49,
               density=1000.0, iterations=10, damping=0.0, global_damping=0.0, xpbd_version=2, femshape=True,
-              noise=0.0, seed=0, box=True, fix=True, tile=None, curve=0.0):
+              noise=0.0, seed=0, box=True, fix=True, tile=None, curve=0.0, rho=0.0, vbd_version=2,
+              target_color=-1):
     import octopusengine_amd as oe
 
     xyz, topo = oe.beam_mesh(element, cells, size)
     rest = curve_rest(xyz, curve) if curve else xyz.copy()
     g = oe.Scene(engine, element, rest, topo, material=material, young=young, poisson=poisson, density=density,
                  distribution=oe.MASS_FEMSHAPE if femshape else oe.MASS_UNIFORM, solver=solver,
-                 iterations=iterations, damping=damping, global_damping=global_damping, xpbd_version=xpbd_version)
+                 iterations=iterations, damping=damping, global_damping=global_damping, xpbd_version=xpbd_version,
+                 rho=rho, vbd_version=vbd_version, target_color=target_color)
     o = O.Scene(element, rest, topo, material=material, young=young, poisson=poisson, density=density,
                 femshape=femshape, solver=solver, iterations=iterations, damping=damping,
                 global_damping=global_damping, xpbd_version=xpbd_version)
+    if solver == 0 and (rho > 0.0 or vbd_version == 3):
+        o.vbd_options(rho=rho, jacobi=vbd_version == 3, target_color=target_color if vbd_version == 3 else -1)
     if fix:
         n = g.fix_plane(FIX_O, FIX_N)
         ids = fixed_ids(rest)

@@ -158,6 +158,31 @@ def test_vbd_curved_elements(engine, case):
         assert err <= TOL1
 
 
+def test_vbd_chebyshev(engine):
+    g, o, _, _ = make_pair(engine, 7, (6, 2, 2), rho=0.7, iterations=6)
+    for _ in range(3):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    err = max_err(g.positions(), o.positions())
+    print(f"VBD Chebyshev rho=0.7 3 steps err={err:.3e}", flush=True)
+    assert err <= TOL1
+
+
+@pytest.mark.parametrize("target", [6, 9])
+def test_vbd_better_coloration_jacobi(engine, target):
+    """Better_Coloration: DSAT reduced to `target` colours (bit-exact with the
+    oracle's), each colour solved from frozen positions then copied."""
+    g, o, _, _ = make_pair(engine, 7, (6, 2, 2), vbd_version=3, target_color=target)
+    assert g.n_colors == target
+    assert np.array_equal(g.colors(), o.colors())
+    for _ in range(2):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    err = max_err(g.positions(), o.positions())
+    print(f"VBD Better_Coloration target={target} err={err:.3e}", flush=True)
+    assert err <= TOL1
+
+
 def test_vbd_graph_replay_matches_eager(engine):
     a, _, _, _ = make_pair(engine, 9, (4, 2, 2))
     b, _, _, _ = make_pair(engine, 9, (4, 2, 2))

@@ -300,6 +300,64 @@ def test_vbd_free_fall():
     assert np.abs(s.positions() - exp).max() < 1e-12
 
 
+def _beam_scene(iterations, **options):
+    xyz, topo = O.beam(O.TETRA10, (3, 2, 1))
+    s = O.Scene(O.TETRA10, xyz, topo, young=1e5, poisson=0.4, iterations=iterations)
+    s.fix(np.nonzero(xyz[:, 0] < 1e-6)[0])
+    s.box()
+    if options:
+        s.vbd_options(**options)
+    return s, xyz, topo
+
+
+def test_vbd_jacobi_equals_gauss_seidel_under_proper_colouring():
+    """No two vertices of one colour share an element in a proper colouring,
+    so solving a colour from frozen positions (v4) and in place (v3) give the
+    same step, bit for bit."""
+    a, _, _ = _beam_scene(4)
+    b, _, _ = _beam_scene(4, jacobi=True)
+    for _ in range(2):
+        a.step(0.005)
+        b.step(0.005)
+    assert np.array_equal(a.positions(), b.positions())
+
+
+@pytest.mark.parametrize("target", [6, 8])
+def test_vbd_reduced_colouring_jacobi_converges_to_the_same_step(target):
+    """ColorReduction to fewer colours leaves same-colour neighbours; the
+    Jacobi sweep over that colouring still converges to the minimiser of the
+    step's incremental potential that Gauss-Seidel reaches."""
+    ref, xyz, topo = _beam_scene(300)
+    ref.step(0.005)
+    j, _, _ = _beam_scene(300, jacobi=True, target_color=target)
+    assert j.ncolors == target
+    assert O.primal_conflicts(O.TETRA10, topo, xyz.shape[0], j.colors()) > 0
+    j.step(0.005)
+    assert np.abs(j.positions() - ref.positions()).max() < 1e-12
+
+
+def test_vbd_chebyshev_keeps_free_fall_and_accelerates():
+    """Chebyshev extrapolation between iterations leaves the exact free-fall
+    step unchanged, and with a spectral radius below the sweep's own it
+    brings a 6-iteration step closer to the converged one."""
+    xyz, topo = O.beam(O.TETRA10, (3, 2, 1))
+    s = O.Scene(O.TETRA10, xyz, topo, young=1e-9, poisson=0.3, iterations=5)
+    s.vbd_options(rho=0.9)
+    s.step(0.01)
+    exp = xyz.astype(np.float64).copy()
+    exp[:, 1] -= 9.81 * 0.01 * 0.01
+    assert np.abs(s.positions() - exp).max() < 1e-12
+    conv, _, _ = _beam_scene(400)
+    plain, _, _ = _beam_scene(6)
+    cheb, _, _ = _beam_scene(6, rho=0.7)
+    for sc in (conv, plain, cheb):
+        sc.step(0.005)
+        sc.step(0.005)
+    e_plain = np.abs(plain.positions() - conv.positions()).max()
+    e_cheb = np.abs(cheb.positions() - conv.positions()).max()
+    assert e_cheb < 0.5 * e_plain
+
+
 def test_vbd_single_free_vertex_is_newton_minimiser():
     """Two fixed-by-constraint layers and one free vertex: VBD with many
     iterations converges to the minimiser of m/(2dt^2)|x-y|^2 + W(x), which is
