@@ -206,6 +206,19 @@ oe_status oe_host_affine_records(int32_t element, const float* xyz, int32_t n_ve
 oe_status oe_host_masses(int32_t element, const float* xyz, int32_t n_vertices, const int32_t* topology,
                          int32_t n_elements, float density, int32_t distribution, float* masses);
 
+/* ------------------------------------------------------------ diagnostics */
+/* Per-element values of the current state, n_elements floats each (any output
+ * may be NULL; synchronising).  Replaces GPU_FEM::get_stress / get_volume /
+ * get_volume_diff / get_inverted (src/GPU/GPU_FEM.cu:117-152):
+ *   stress = sum_q vonMises(cauchy(F_q)) V_q, volume = sum_q |det F_q| V_q,
+ *   volume_diff = volume - sum_q V_q, inverted = 1 if sum_q det F_q w_q <= -1e-6. */
+oe_status oe_scene_element_diagnostics(oe_scene* s, float* stress, float* volume, float* volume_diff,
+                                       float* inverted);
+/* Residual force of each free vertex's local VBD problem at the current state
+ * (3 * n_vertices floats; fixed vertices 0).  Replaces GPU_VBD_FEM::get_forces
+ * (src/GPU/VBD/GPU_VBD_FEM.cu:482-559). */
+oe_status oe_scene_residual(oe_scene* s, float dt, float* r);
+
 /* ------------------------------------------------------------ VTK output */
 /* Legacy-VTK 5.1 ASCII unstructured grid in the layout of the reference's
  * VTK_Formater (src/Mesh/Converter/VTK_Formater.cpp): open, mesh (POINTS,

@@ -260,6 +260,19 @@ class Scene:
         L.check(L.lib.oe_scene_get_tile(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, bool(c.value)
 
+    def element_diagnostics(self):
+        """dict of per-element stress (von Mises x V), volume, volume_diff, inverted."""
+        out = {k: np.empty(self.n_elements, np.float32) for k in ("stress", "volume", "volume_diff", "inverted")}
+        L.check(L.lib.oe_scene_element_diagnostics(self._h, *(L.fptr(out[k]) for k in
+                                                              ("stress", "volume", "volume_diff", "inverted"))))
+        return out
+
+    def residual(self, dt: float):
+        """Residual force of every free vertex's local VBD problem, (N, 3)."""
+        r = np.empty((self.n_vertices, 3), np.float32)
+        L.check(L.lib.oe_scene_residual(self._h, dt, L.fptr(r)))
+        return r
+
     def save_vtk(self, path):
         """Rest geometry + point data "weights" (masses) and "u" (displacement)."""
         L.check(L.lib.oe_scene_save_vtk(self._h, str(path).encode()))

@@ -590,6 +590,25 @@ oe_status oe_host_masses(int32_t element, const float* xyz, int32_t nv, const in
     });
 }
 
+// ------------------------------------------------------------------ diagnostics
+oe_status oe_scene_element_diagnostics(oe_scene* s, float* stress, float* volume, float* volume_diff,
+                                       float* inverted) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        s->scene->fem->diagnostics(s->scene->ps.get(), stress, volume, volume_diff, inverted);
+    });
+}
+
+oe_status oe_scene_residual(oe_scene* s, float dt, float* r) {
+    return guard([&] {
+        OE_REQUIRE(s && r, "bad argument");
+        OE_REQUIRE(s->scene->vbd_fem, "the residual is defined for VBD scenes");
+        std::vector<float> h;
+        s->scene->vbd_fem->residual(s->scene->ps.get(), dt, h);
+        std::memcpy(r, h.data(), h.size() * sizeof(float));
+    });
+}
+
 // ------------------------------------------------------------------ VTK I/O
 oe_status oe_vtk_open(const char* path, oe_vtk** out) {
     return guard([&] {

@@ -91,6 +91,11 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
         a.ps.f[vid] = f;
         return;
     }
+    if (a.mode == GATHER_VBD_RESIDUAL) {
+        if (lane == 0)
+            a.p_new[vid] = vbd_residual(acc, P[vid], a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt);
+        return;
+    }
     float du[3] = {0.f, 0.f, 0.f};
     if (lane == 0) {
         const float4 pv = P[vid];

@@ -12,6 +12,7 @@ enum GatherMode : int {
     GATHER_VBD_INPLACE = 0,     // Gauss-Seidel inside a proper colour (reference v3)
     GATHER_VBD_JACOBI = 1,      // write p_new, copied after the colour (reference v4)
     GATHER_EXPLICIT_FORCE = 2,  // internal force only, overwrite f (explicit integrator)
+    GATHER_VBD_RESIDUAL = 3,    // residual force of the local VBD problem into p_new (no update)
 };
 
 struct GatherArgs {
@@ -20,7 +21,7 @@ struct GatherArgs {
     OwnersView own;
     MaterialView mt;
     const float4* y;   // inertia target (VBD)
-    float4* p_new;     // Jacobi output (VBD v4)
+    float4* p_new;     // Jacobi output (VBD v4) or residual forces
     // Displacement gradients Ju(e, q) = sum_k u_k (x) dN_k(q), 9 floats per
     // (element, quadrature point), or null to rebuild them from the nodes.
     // When set (in-place VBD only) the gather reads Ju instead of the S node

@@ -15,6 +15,7 @@
 #include <numeric>
 #include <stdexcept>
 
+#include "oe_diag.hpp"
 #include "oe_gather.hpp"
 #include "oe_xpbd.hpp"
 
@@ -214,6 +215,20 @@ GPU_FEM::GPU_FEM(Engine* e, Element el, const HostMesh& mesh, float young, float
     elem_volume = std::move(fc.elem_volume);
 }
 
+void GPU_FEM::diagnostics(GPU_ParticleSystem* ps, float* stress, float* volume, float* volume_diff, float* inverted) {
+    DBuf<float> out(&eng->pool, static_cast<size_t>(E) * 4);
+    float* base = out.get();
+    if (!launch_element_diagnostics(ps->get_parameters(), fem_view(), mt, base, base + E, base + 2 * static_cast<size_t>(E),
+                                    base + 3 * static_cast<size_t>(E), eng->stream))
+        throw std::invalid_argument("unsupported element");
+    OE_LAUNCH_CHECK();
+    std::vector<float> h;
+    out.download(h, eng->stream);
+    float* dst[4] = {stress, volume, volume_diff, inverted};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) std::memcpy(dst[k], h.data() + static_cast<size_t>(k) * E, sizeof(float) * E);
+}
+
 FemView GPU_FEM::fem_view() const {
     return FemView{E, S, Q, topo.get(), jxv.get(), dN.get(), affine ? 1 : 0, wq.get()};
 }
@@ -263,6 +278,39 @@ GPU_VBD_FEM::~GPU_VBD_FEM() {
 float* GPU_VBD_FEM::ensure_ju() {
     if (!ju.get()) ju.allocate(&eng->pool, static_cast<size_t>(E) * Q * 9);
     return ju.get();
+}
+
+void GPU_VBD_FEM::residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r) {
+    if (!y) throw std::logic_error("GPU_VBD_FEM used outside a GPU_VBD integrator");
+    if (!(dt > 0.f)) throw std::invalid_argument("residual needs dt > 0");
+    const int n = ps->nb_particles();
+    if (!d_residual.get()) {
+        d_residual.allocate(&eng->pool, n);
+        OE_HIP(hipMemsetAsync(d_residual.get(), 0, sizeof(float4) * n, eng->stream));
+    }
+    GatherArgs a{};
+    a.ps = ps->get_parameters();
+    a.fem = fem_view();
+    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
+    a.mt = mt;
+    a.y = y;
+    a.p_new = d_residual.get();
+    a.ju = nullptr;
+    a.damping = damping;
+    a.dt = dt;
+    a.first_slot = 0;
+    a.count = n;
+    a.mode = GATHER_VBD_RESIDUAL;
+    if (!launch_gather(S, Q, tile.L, tile.QL, a, eng->stream)) throw std::invalid_argument("unsupported gather tile");
+    OE_LAUNCH_CHECK();
+    std::vector<float4> h;
+    d_residual.download(h, eng->stream);
+    r.resize(3 * static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        r[3 * i] = h[i].x;
+        r[3 * i + 1] = h[i].y;
+        r[3 * i + 2] = h[i].z;
+    }
 }
 
 void GPU_VBD_FEM::set_shuffle_seed(uint64_t seed) {

@@ -110,6 +110,9 @@ struct GPU_PBD final : GPU_Integrator {
 struct GPU_FEM : GPU_Dynamic {
     GPU_FEM(Engine* eng, Element element, const HostMesh& mesh, float young, float poisson, Material material);
     FemView fem_view() const;
+    // Per-element stress * V, volume, volume difference and inverted flag of
+    // the current state (E floats each, any may be null; synchronising).
+    void diagnostics(GPU_ParticleSystem* ps, float* stress, float* volume, float* volume_diff, float* inverted);
     Engine* eng;
     Element element;
     int S, Q, E;
@@ -155,6 +158,10 @@ struct GPU_VBD_FEM final : GPU_FEM {
     DBuf<float4> p_new;
     DBuf<float> ju;           // cached displacement gradients, E * Q * 9 (allocated on first use)
     float* ensure_ju();
+    // Residual force of every free vertex's local problem at the current
+    // positions (3N floats; fixed vertices keep 0; synchronising).
+    void residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r);
+    DBuf<float4> d_residual;
     bool cached() const { return tile.cache && version != Better_Coloration; }
     int64_t conflicts = 0;
     // optional per-launch timing of the gather kernel (events around each colour)

@@ -156,6 +156,27 @@ __device__ __forceinline__ void nh_contrib(const float Jx[9], const float JXi[9]
     a.v[8] += d + l * cg[2] * cg[2];
 }
 
+// Residual force of the local VBD problem at the current position
+// (reference kernel_vbd_compute_residual, src/GPU/VBD/GPU_VBD_FEM.cu:482-542):
+//   r = f - (k/dt) H (p - last_p) - (m/dt^2)(p - y)
+__device__ __forceinline__ float4 vbd_residual(const Acc9& a, float4 p, float4 lp, float4 y, float m, float damping,
+                                               float dt) {
+    double r[3] = {a.v[0], a.v[1], a.v[2]};
+    if (damping != 0.f) {
+        const double k = static_cast<double>(damping) / dt;
+        const double u0 = static_cast<double>(p.x) - lp.x, u1 = static_cast<double>(p.y) - lp.y,
+                     u2 = static_cast<double>(p.z) - lp.z;
+        r[0] -= k * (a.v[3] * u0 + a.v[4] * u1 + a.v[5] * u2);
+        r[1] -= k * (a.v[4] * u0 + a.v[6] * u1 + a.v[7] * u2);
+        r[2] -= k * (a.v[5] * u0 + a.v[7] * u1 + a.v[8] * u2);
+    }
+    const double mh2 = static_cast<double>(m) / (static_cast<double>(dt) * dt);
+    r[0] -= mh2 * (static_cast<double>(p.x) - y.x);
+    r[1] -= mh2 * (static_cast<double>(p.y) - y.y);
+    r[2] -= mh2 * (static_cast<double>(p.z) - y.z);
+    return make_float4(static_cast<float>(r[0]), static_cast<float>(r[1]), static_cast<float>(r[2]), 0.f);
+}
+
 // Newton step of the local VBD problem in double precision
 // (reference compute_correction, src/GPU/VBD/GPU_VBD_FEM.cu:107-122):
 //   f -= (k/dt) H (p - last_p);  H += (k/dt) H

@@ -1278,6 +1278,64 @@ static void vbd_substep(orc_scene* s, double dt) {
     for (i = 0; i < 3 * s->N; ++i) s->v[i] = (s->p[i] - s->last_p[i]) / dt;
 }
 
+/* Per-element diagnostics (GPU_FEM.cu:20-115 with Jx rebuilt at every
+ * quadrature point and JX_inv indexed per point):
+ * stress = sum_q vonMises(P F^T / det F) V_q (von_mises_stress,
+ * GPU_FEM_Material.cu:219-230), volume = sum_q |det Jx_q| w_q, volume_diff =
+ * volume - sum_q V_q, inverted = sum_q det(F_q) w_q <= -1e-6. */
+void orc_scene_diagnostics(orc_scene* s, double* stress, double* volume, double* vdiff, double* inverted) {
+    int e, q, a, b, k;
+    for (e = 0; e < s->E; ++e) {
+        double st = 0, vo = 0, vd = 0, sv = 0;
+        for (q = 0; q < s->Q; ++q) {
+            double F[9], P[9], H[9], sg[9], g[3] = {0, 0, 0}, J, w;
+            const double V = s->V[(size_t)e * s->Q + q];
+            kinematics(s, e, q, 0, F, g);
+            orc_vbd_material(s->material, s->lambda, s->mu, F, g, V, P, H);
+            J = det3(F);
+            for (a = 0; a < 3; ++a)       /* column */
+                for (b = 0; b < 3; ++b) { /* row */
+                    double t = 0;
+                    for (k = 0; k < 3; ++k) t += P[k * 3 + b] * F[k * 3 + a];
+                    sg[a * 3 + b] = t / J;
+                }
+            {
+                const double d01 = sg[0] - sg[4], d12 = sg[4] - sg[8], d02 = sg[0] - sg[8];
+                st += sqrt(0.5 * (d01 * d01 + d12 * d12 + d02 * d02) +
+                           3.0 * (sg[1] * sg[1] + sg[5] * sg[5] + sg[2] * sg[2])) * V;
+            }
+            w = s->wq[q];
+            /* |det Jx| w = |det F| |det J_X| w = |det F| V */
+            vo += fabs(J) * V;
+            vd += fabs(J) * V - V;
+            sv += J * w;
+        }
+        if (stress) stress[e] = st;
+        if (volume) volume[e] = vo;
+        if (vdiff) vdiff[e] = vd;
+        if (inverted) inverted[e] = sv > -1e-6 ? 0.0 : 1.0;
+    }
+}
+
+/* kernel_vbd_compute_residual (GPU_VBD_FEM.cu:482-542): free vertices get
+ * f - (k/dt) H (p - last_p) - (m/dt^2)(p - y); fixed vertices 0. */
+void orc_scene_residual(orc_scene* s, double dt, double* r) {
+    int v, k;
+    for (v = 0; v < s->N; ++v) {
+        double f[3], H[9];
+        const double kd = s->damping / dt, mh2 = s->m[v] / (dt * dt);
+        for (k = 0; k < 3; ++k) r[3 * v + k] = 0;
+        if (s->mask[v] == 0) continue;
+        vertex_fH(s, v, f, H);
+        if (s->damping != 0.0) {
+            const double u[3] = {s->p[3 * v] - s->last_p[3 * v], s->p[3 * v + 1] - s->last_p[3 * v + 1],
+                                 s->p[3 * v + 2] - s->last_p[3 * v + 2]};
+            for (k = 0; k < 3; ++k) f[k] -= kd * (H[k] * u[0] + H[3 + k] * u[1] + H[6 + k] * u[2]);
+        }
+        for (k = 0; k < 3; ++k) r[3 * v + k] = f[k] - mh2 * (s->p[3 * v + k] - s->y[3 * v + k]);
+    }
+}
+
 /* kenerl_semi_exicit_integration (GPU_Integrator.cu:27-34) */
 static void semi_implicit(orc_scene* s, double dt) {
     int i, k;

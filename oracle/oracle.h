@@ -64,6 +64,10 @@ void orc_scene_box(orc_scene* s, const double pmin[3], const double pmax[3]);
 /* VBD options: Chebyshev radius (0 = off), Jacobi colour sweep (v4), colour
  * reduction target (> 0); returns the colour count */
 int orc_scene_vbd_options(orc_scene* s, double rho, int jacobi, int target_color);
+/* per-element stress (von Mises x V), volume, volume difference, inverted flag */
+void orc_scene_diagnostics(orc_scene* s, double* stress, double* volume, double* vdiff, double* inverted);
+/* residual force of every free vertex's local VBD problem (3N) */
+void orc_scene_residual(orc_scene* s, double dt, double* r);
 /* colour-order override (VBD): ncolor entries, NULL = ascending */
 void orc_scene_color_order(orc_scene* s, const int* order);
 void orc_scene_step(orc_scene* s, double dt, int substeps);

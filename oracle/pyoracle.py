@@ -59,6 +59,8 @@ lib.orc_scene_fix.argtypes = [P, i32p, C.c_int]
 lib.orc_scene_box.argtypes = [P, f64p, f64p]
 lib.orc_scene_color_order.argtypes = [P, i32p]
 lib.orc_scene_vbd_options.argtypes = [P, C.c_double, C.c_int, C.c_int]
+lib.orc_scene_diagnostics.argtypes = [P, f64p, f64p, f64p, f64p]
+lib.orc_scene_residual.argtypes = [P, C.c_double, f64p]
 lib.orc_scene_step.argtypes = [P, C.c_double, C.c_int]
 lib.orc_scene_get.argtypes = [P, C.c_int, f64p]
 lib.orc_scene_set.argtypes = [P, C.c_int, f64p]
@@ -210,6 +212,16 @@ class Scene:
         if n < 0:
             raise ValueError("VBD options on a non-VBD scene")
         return n
+
+    def element_diagnostics(self):
+        out = {k: np.empty(self.E) for k in ("stress", "volume", "volume_diff", "inverted")}
+        lib.orc_scene_diagnostics(self._h, *(dp(out[k]) for k in ("stress", "volume", "volume_diff", "inverted")))
+        return out
+
+    def residual(self, dt):
+        r = np.empty((self.N, 3))
+        lib.orc_scene_residual(self._h, dt, dp(r))
+        return r
 
     def color_order(self, order):
         if order is None:

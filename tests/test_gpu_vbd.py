@@ -183,6 +183,31 @@ def test_vbd_better_coloration_jacobi(engine, target):
     assert err <= TOL1
 
 
+@pytest.mark.parametrize("case", [(3, (8, 2, 2)), (7, (6, 2, 2)), (9, (4, 2, 2)), (8, (3, 1, 1))],
+                         ids=lambda c: f"el{c[0]}")
+def test_element_diagnostics_and_residual(engine, case):
+    """Per-element stress / volume / volume_diff / inverted and the VBD
+    residual forces after a few (unconverged) steps, vs the oracle."""
+    el, cells = case
+    g, o, _, _ = make_pair(engine, el, cells, iterations=3, young=1e5, poisson=0.45)
+    for _ in range(3):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    dg, do = g.element_diagnostics(), o.element_diagnostics()
+    for k in ("stress", "volume", "volume_diff"):
+        scale = float(np.abs(do[k]).max())
+        err = float(np.abs(dg[k] - do[k]).max())
+        print(f"el={el} {k}: max {scale:.3e} err {err:.3e}", flush=True)
+        tol = 1e-4 * scale if k == "stress" else 1e-5 * float(np.abs(do["volume"]).max())
+        assert err <= tol
+    assert np.array_equal(dg["inverted"], do["inverted"].astype(np.float32))
+    rg, ro = g.residual(0.005), o.residual(0.005)
+    err = float(np.abs(rg - ro).max())
+    print(f"el={el} residual max {np.abs(ro).max():.3e} err {err:.3e}", flush=True)
+    assert np.abs(ro).max() > 0
+    assert err <= 2e-2 * float(np.abs(ro).max())
+
+
 def test_vbd_graph_replay_matches_eager(engine):
     a, _, _, _ = make_pair(engine, 9, (4, 2, 2))
     b, _, _, _ = make_pair(engine, 9, (4, 2, 2))

@@ -358,6 +358,58 @@ def test_vbd_chebyshev_keeps_free_fall_and_accelerates():
     assert e_cheb < 0.5 * e_plain
 
 
+@pytest.mark.parametrize("el,cells", [(O.TETRA, (4, 2, 2)), (O.TETRA10, (3, 2, 1)), (O.HEXA27, (2, 1, 1))])
+@pytest.mark.parametrize("mat", [O.NEOHOOKE, O.STABLE_NEOHOOKE])
+def test_element_diagnostics_closed_forms(el, cells, mat):
+    """Rest: no stress, volume = sum V, nothing inverted.  Uniaxial stretch
+    F = diag(a, 1, 1): Cauchy stress sigma_xx - sigma_yy = mu (a - 1/a) for
+    both Neo-Hookean forms (the lambda terms are equal on the diagonal), so
+    stress = mu |a - 1/a| V_e, volume = a V_e.  A reflection inverts every
+    element and keeps the volume."""
+    xyz, topo = O.beam(el, cells)
+    s = O.Scene(el, xyz, topo, material=mat, young=1e5, poisson=0.4)
+    _, V = O.fem_constants(el, xyz, topo)
+    Ve = V.sum(1)
+    d = s.element_diagnostics()
+    assert np.abs(d["stress"]).max() < 1e-9
+    assert np.abs(d["volume"] - Ve).max() < 1e-15
+    assert np.abs(d["volume_diff"]).max() < 1e-15 and d["inverted"].max() == 0
+    a = 1.3
+    mu = 1e5 / 2.8
+    p = xyz.astype(np.float64).copy()
+    p[:, 0] *= a
+    s.set("p", p)
+    d = s.element_diagnostics()
+    assert np.allclose(d["stress"], mu * abs(a - 1 / a) * Ve, rtol=1e-12)
+    assert np.allclose(d["volume"], a * Ve, rtol=1e-12)
+    assert np.allclose(d["volume_diff"], (a - 1) * Ve, rtol=1e-12, atol=1e-18)
+    p = xyz.astype(np.float64).copy()
+    p[:, 0] *= -1
+    s.set("p", p)
+    d = s.element_diagnostics()
+    assert d["inverted"].min() == 1
+    assert np.allclose(d["volume"], Ve, rtol=1e-12)
+
+
+def test_residual_vanishes_as_vbd_converges():
+    """The residual force of the local problems is the gradient of the step's
+    incremental potential: it shrinks with more iterations and vanishes at the
+    converged step; fixed vertices report zero."""
+    xyz, topo = O.beam(O.TETRA10, (3, 2, 1))
+    fixed = np.nonzero(xyz[:, 0] < 1e-6)[0]
+    res = []
+    for it in (2, 20, 300):
+        s = O.Scene(O.TETRA10, xyz, topo, young=1e5, poisson=0.4, iterations=it)
+        s.fix(fixed)
+        s.box()
+        s.step(0.005)
+        s.step(0.005)
+        r = s.residual(0.005)
+        assert np.abs(r[fixed]).max() == 0
+        res.append(np.abs(r).max())
+    assert res[0] > 1e-2 and res[1] < res[0] * 1e-2 and res[2] < 1e-9
+
+
 def test_vbd_single_free_vertex_is_newton_minimiser():
     """Two fixed-by-constraint layers and one free vertex: VBD with many
     iterations converges to the minimiser of m/(2dt^2)|x-y|^2 + W(x), which is
