@@ -219,6 +219,19 @@ oe_status oe_scene_element_diagnostics(oe_scene* s, float* stress, float* volume
  * (src/GPU/VBD/GPU_VBD_FEM.cu:482-559). */
 oe_status oe_scene_residual(oe_scene* s, float dt, float* r);
 
+/* ------------------------------------------------------------ mesh files */
+/* Legacy-VTK ASCII unstructured grids (.vtk) and Gmsh 2.2 ASCII (.msh), as the
+ * reference's VTK_Loader / Msh_Loader read them
+ * (src/Mesh/Converter/MeshLoader.cpp:27-237).  counts[element id] (10
+ * entries) = number of elements of that type; topologies are in the engine's
+ * (the reference's) node order. */
+typedef struct oe_mesh oe_mesh;
+oe_status oe_mesh_load(const char* path, oe_mesh** out);
+oe_status oe_mesh_info(oe_mesh* m, int32_t* n_vertices, int32_t* counts);
+oe_status oe_mesh_vertices(oe_mesh* m, float* xyz);
+oe_status oe_mesh_topology(oe_mesh* m, int32_t element, int32_t* topology);
+oe_status oe_mesh_destroy(oe_mesh* m);
+
 /* ------------------------------------------------------------ VTK output */
 /* Legacy-VTK 5.1 ASCII unstructured grid in the layout of the reference's
  * VTK_Formater (src/Mesh/Converter/VTK_Formater.cpp): open, mesh (POINTS,

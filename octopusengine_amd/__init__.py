@@ -114,6 +114,28 @@ def host_masses(element: int, xyz, topo, density: float, distribution: int):
     return out
 
 
+def load_mesh(path):
+    """(xyz (N, 3) float32, {element: topology (E, S) int32}) from a .vtk or
+    .msh file (the reference's VTK_Loader / Msh_Loader formats)."""
+    h = C.c_void_p()
+    L.check(L.lib.oe_mesh_load(str(path).encode(), C.byref(h)))
+    try:
+        nv = C.c_int32()
+        counts = np.zeros(10, np.int32)
+        L.check(L.lib.oe_mesh_info(h, C.byref(nv), L.iptr(counts)))
+        xyz = np.empty((nv.value, 3), np.float32)
+        L.check(L.lib.oe_mesh_vertices(h, L.fptr(xyz)))
+        topos = {}
+        for el in (TETRA, HEXA, TETRA10, TETRA20, HEXA27):
+            if counts[el]:
+                t = np.empty((int(counts[el]), NODES[el]), np.int32)
+                L.check(L.lib.oe_mesh_topology(h, el, L.iptr(t)))
+                topos[el] = t
+        return xyz, topos
+    finally:
+        L.lib.oe_mesh_destroy(h)
+
+
 class VtkWriter:
     """Legacy-VTK 5.1 ASCII writer with the reference's file layout
     (VTK_Formater: open, save_mesh, start_point_data / start_cell_data,

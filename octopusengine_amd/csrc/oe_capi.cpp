@@ -29,6 +29,9 @@ struct oe_server {
 struct oe_vtk {
     std::unique_ptr<oe::VtkWriter> w;
 };
+struct oe_mesh {
+    oe::LoadedMesh m;
+};
 
 namespace {
 
@@ -607,6 +610,48 @@ oe_status oe_scene_residual(oe_scene* s, float dt, float* r) {
         s->scene->vbd_fem->residual(s->scene->ps.get(), dt, h);
         std::memcpy(r, h.data(), h.size() * sizeof(float));
     });
+}
+
+// ------------------------------------------------------------------ mesh files
+oe_status oe_mesh_load(const char* path, oe_mesh** out) {
+    return guard([&] {
+        OE_REQUIRE(path && out, "bad argument");
+        auto m = std::make_unique<oe_mesh>();
+        m->m = oe::load_mesh(path);
+        *out = m.release();
+    });
+}
+
+oe_status oe_mesh_info(oe_mesh* m, int32_t* n_vertices, int32_t* counts) {
+    return guard([&] {
+        OE_REQUIRE(m, "bad argument");
+        if (n_vertices) *n_vertices = static_cast<int32_t>(m->m.xyz.size() / 3);
+        if (counts) {
+            for (int e = 0; e < 10; ++e) counts[e] = 0;
+            for (const auto& kv : m->m.topologies)
+                counts[kv.first] = static_cast<int32_t>(kv.second.size() / oe::elem_nb_vertices(kv.first));
+        }
+    });
+}
+
+oe_status oe_mesh_vertices(oe_mesh* m, float* xyz) {
+    return guard([&] {
+        OE_REQUIRE(m && xyz, "bad argument");
+        std::memcpy(xyz, m->m.xyz.data(), m->m.xyz.size() * sizeof(float));
+    });
+}
+
+oe_status oe_mesh_topology(oe_mesh* m, int32_t element, int32_t* topology) {
+    return guard([&] {
+        OE_REQUIRE(m && topology && oe::element_supported(element), "bad argument");
+        const auto it = m->m.topologies.find(static_cast<oe::Element>(element));
+        if (it != m->m.topologies.end())
+            std::memcpy(topology, it->second.data(), it->second.size() * sizeof(int32_t));
+    });
+}
+
+oe_status oe_mesh_destroy(oe_mesh* m) {
+    return guard([&] { delete m; });
 }
 
 // ------------------------------------------------------------------ VTK I/O

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <fstream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -143,6 +144,17 @@ struct ElementSchedule {
     std::vector<int32_t> color_begin;   // nb_color + 1
 };
 ElementSchedule build_element_schedule(const Coloration& col, int nb_elements);
+
+// Mesh files (reference VTK_Loader / Msh_Loader,
+// src/Mesh/Converter/MeshLoader.cpp:27-237): float coordinates and one
+// topology per element type, in the engine's node order.
+struct LoadedMesh {
+    std::vector<float> xyz;
+    std::map<Element, std::vector<int32_t>> topologies;
+};
+LoadedMesh load_vtk(const std::string& path);
+LoadedMesh load_msh(const std::string& path);
+LoadedMesh load_mesh(const std::string& path);   // by extension: .vtk or .msh
 
 // Legacy-VTK 5.1 ASCII unstructured-grid writer with the reference's layout
 // (VTK_Formater, src/Mesh/Converter/VTK_Formater.cpp): mesh(), then
