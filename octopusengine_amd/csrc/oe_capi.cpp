@@ -245,7 +245,10 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t cac
         check_tile(sc.fem->Q, lanes, qpl, cache);
         OE_REQUIRE(cache == 0 || sc.vbd_fem, "the gradient cache is a VBD option");
         sc.set_graph(false);
-        if (sc.vbd_fem) sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, cache != 0};
+        if (sc.vbd_fem) {
+            sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, cache != 0};
+            if (sc.vbd_fem->cached()) sc.vbd_fem->ensure_ju();
+        }
         if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, false};
     });
 }

@@ -269,6 +269,8 @@ GPU_VBD_FEM::GPU_VBD_FEM(Engine* e, Element el, const HostMesh& mesh, Material m
     owners = build_owners(mesh, colors);
     upload_owners(eng, owners, d_slot_vid, d_slot_nbo, d_own);
     if (version == Better_Coloration) p_new.allocate(&eng->pool, mesh.nb_vertices());
+    // allocated here, not in step(), which may run under HIP graph capture
+    if (cached()) ensure_ju();
 }
 
 GPU_VBD_FEM::~GPU_VBD_FEM() {
@@ -339,7 +341,8 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
     a.ju = nullptr;
     if (cached()) {
         // gradients of this iteration's starting positions
-        a.ju = ensure_ju();
+        if (!ju.get()) throw std::logic_error("gradient cache not allocated");
+        a.ju = ju.get();
         if (!launch_ju_refresh(S, Q, a.ps, a.fem, a.ju, s)) throw std::invalid_argument("unsupported element");
     }
     for (int c : order) {
