@@ -108,13 +108,16 @@ oe_status oe_scene_box_limit(oe_scene* s, const float pmin[3], const float pmax[
 oe_status oe_scene_step(oe_scene* s, float dt, int32_t substeps);
 /* Capture the substep loop into a HIP graph and replay it (1) or launch eagerly (0). */
 oe_status oe_scene_set_graph(oe_scene* s, int32_t enable);
-/* Gather tile: lanes per vertex (4..64, power of two), quadrature points per
- * lane, and cache = 1 to keep the displacement gradients of every (element,
- * quadrature point) in device memory during a VBD sweep (qpl must be 1; VBD
- * scenes only).  No counterpart in the reference, whose kernels fix the
- * thread layout per version (src/GPU/VBD/GPU_VBD_FEM.cu:712-756). */
-oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t cache);
-oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* cache);
+/* VBD colour-pass layout: lanes per vertex (4..64, power of two) and
+ * quadrature points per lane of the gather kernel, and mode = 0 (gather),
+ * 1 (gather with the displacement gradients of every (element, quadrature
+ * point) kept in device memory during a sweep; qpl must be 1) or 2 (element
+ * tasks with sum-factorised gradients then vertex solves; 27-node hexahedra
+ * only).  Modes 1 and 2 are VBD options.  No counterpart in the reference,
+ * whose kernels fix the thread layout per version
+ * (src/GPU/VBD/GPU_VBD_FEM.cu:712-756). */
+oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mode);
+oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* mode);
 /* Element record layout chosen at creation: affine = 1 when every element is
  * an affine image of the reference element and its J^-1, |det J| are stored
  * once per element; 0 for one record per (element, quadrature point) as in
@@ -150,7 +153,7 @@ oe_status oe_batch_create(oe_engine* e, oe_scene* const* scenes, int32_t n, oe_b
 oe_status oe_batch_destroy(oe_batch* b);
 oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps);
 oe_status oe_batch_set_graph(oe_batch* b, int32_t enable);
-oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t cache);
+oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t mode);
 oe_status oe_batch_set_timing(oe_batch* b, int32_t enable);
 oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches);
 /* Gather launches and vertex-updates per substep; algorithmic gather bytes per substep. */

@@ -175,6 +175,10 @@ class VtkWriter:
             L.check(L.lib.oe_vtk_close(h))
 
 
+# VBD colour-pass modes (oe_scene_set_tile)
+PASS_GATHER, PASS_CACHE, PASS_TASKS = 0, 1, 2
+
+
 # ------------------------------------------------------------------ device side
 class Engine:
     """One GPU, one stream, one device-memory budget (bytes, 0 = none)."""
@@ -271,16 +275,17 @@ class Scene:
     def set_graph(self, enable: bool):
         L.check(L.lib.oe_scene_set_graph(self._h, int(enable)))
 
-    def set_tile(self, lanes: int, qpl: int, cache: bool = False):
-        """Gather tile: lanes per vertex, quadrature points per lane, and
-        whether the VBD sweep caches the displacement gradients (qpl = 1)."""
-        L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl, int(cache)))
+    def set_tile(self, lanes: int, qpl: int, mode: int = PASS_GATHER):
+        """VBD colour-pass layout: lanes per vertex, quadrature points per lane
+        and mode (PASS_GATHER, PASS_CACHE: cached gradients with qpl = 1,
+        PASS_TASKS: element tasks, 27-node hexahedra)."""
+        L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl, int(mode)))
 
     def get_tile(self):
-        """(lanes, quadrature points per lane, gradient cache on)."""
+        """(lanes, quadrature points per lane, mode)."""
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         L.check(L.lib.oe_scene_get_tile(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, bool(c.value)
+        return a.value, b.value, c.value
 
     def element_diagnostics(self):
         """dict of per-element stress (von Mises x V), volume, volume_diff, inverted."""
@@ -388,8 +393,8 @@ class Batch:
     def set_graph(self, enable: bool):
         L.check(L.lib.oe_batch_set_graph(self._h, int(enable)))
 
-    def set_tile(self, element: int, lanes: int, qpl: int, cache: bool = False):
-        L.check(L.lib.oe_batch_set_tile(self._h, element, lanes, qpl, int(cache)))
+    def set_tile(self, element: int, lanes: int, qpl: int, mode: int = PASS_GATHER):
+        L.check(L.lib.oe_batch_set_tile(self._h, element, lanes, qpl, int(mode)))
 
     def set_timing(self, enable: bool):
         L.check(L.lib.oe_batch_set_timing(self._h, int(enable)))

@@ -1,5 +1,7 @@
 #include "oe_batch.hpp"
 
+#include "oe_q2.hpp"
+
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -24,6 +26,9 @@ GPU_Box_Limit* find_box(const Scene& s) {
 std::map<int, GatherTile> default_tiles() {
     std::map<int, GatherTile> t;
     for (Element e : {Tetra, Hexa, Tetra10, Tetra20, Hexa27}) t[e] = default_tile(e);
+    // many small P2 scenes per launch favour narrower vertex groups than one
+    // large scene does (measured on the 256-scene batch)
+    t[Tetra10] = GatherTile{8, 2, PASS_GATHER};
     return t;
 }
 
@@ -56,7 +61,11 @@ GatherTile tile_of(const std::map<int, GatherTile>& tiles, Element el) {
 SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int, GatherTile>& tiles) {
     check_batchable(s);
     SceneDev d{};
-    d.ju = tile_of(tiles, s.mesh.element).cache ? s.vbd_fem->ensure_ju() : nullptr;
+    const GatherTile t = tile_of(tiles, s.mesh.element);
+    if (t.mode == PASS_TASKS && !s.vbd_fem->has_q2_basis)
+        throw std::invalid_argument("the element-task pass is for 27-node hexahedra");
+    d.ju = t.mode == PASS_CACHE ? s.vbd_fem->ensure_ju() : nullptr;
+    d.partial = t.mode == PASS_TASKS ? s.vbd_fem->ensure_partial() : nullptr;
     d.ps = s.ps->get_parameters();
     d.fem = s.fem->fem_view();
     d.own = OwnersView{s.vbd_fem->d_slot_vid.get(), s.vbd_fem->d_slot_nbo.get(), s.vbd_fem->d_own.get()};
@@ -89,6 +98,8 @@ void Schedule::clear() {
     glaunches.clear();
     vsegs.clear();
     vblock_seg.clear();
+    tsegs.clear();
+    tblock_seg.clear();
     jlaunches.clear();
     jsegs.clear();
     jblock_seg.clear();
@@ -128,17 +139,31 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     gl.Q = s->fem->Q;
                     gl.L = t.L;
                     gl.QL = t.QL;
-                    gl.cache = t.cache;
+                    gl.mode = t.mode;
+                    gl.tblock_base = static_cast<int>(tblock_seg.size());
                     any = true;
+                    if (gl.mode == PASS_TASKS) q2_basis = s->vbd_fem->q2_basis;
                 }
-                const int nb = static_cast<int>((static_cast<int64_t>(count) * gl.L + kBlock - 1) / kBlock);
+                // gather: L lanes per slot; tasks: the vertex solve takes one thread per slot
+                const int lanes = gl.mode == PASS_TASKS ? 1 : gl.L;
+                const int nb = static_cast<int>((static_cast<int64_t>(count) * lanes + kBlock - 1) / kBlock);
                 const int sid = static_cast<int>(segs.size());
                 segs.push_back(BatchSeg{table_index[i], first, count, static_cast<int>(block_seg.size())});
                 block_seg.insert(block_seg.end(), nb, sid);
+                if (gl.mode == PASS_TASKS) {
+                    const int tasks = static_cast<int>(s->vbd_fem->owners.color_incidences[c]);
+                    const int tpb = q2_tasks_per_block();
+                    const int tnb = (tasks + tpb - 1) / tpb;
+                    const int tid = static_cast<int>(tsegs.size());
+                    tsegs.push_back(BatchSeg{table_index[i], s->vbd_fem->owners.slot_off[first], tasks,
+                                             static_cast<int>(tblock_seg.size())});
+                    tblock_seg.insert(tblock_seg.end(), tnb, tid);
+                }
                 gl.slots += count;
                 gl.incidences += s->vbd_fem->owners.color_incidences[c];
             }
             gl.nblocks = static_cast<int>(block_seg.size()) - gl.block_base;
+            gl.tnblocks = static_cast<int>(tblock_seg.size()) - gl.tblock_base;
             if (any) glaunches.push_back(gl);
         }
     }
@@ -150,22 +175,25 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
         vblock_seg.insert(vblock_seg.end(), nb, sid);
     }
     for (Element el : order) {
-        if (!tile_of(tiles, el).cache) continue;
+        if (tile_of(tiles, el).mode != PASS_CACHE) continue;
         JLaunch jl{};
         jl.block_base = static_cast<int>(jblock_seg.size());
         bool any = false;
         for (size_t i = 0; i < scenes.size(); ++i) {
             Scene* s = scenes[i];
             if (s->mesh.element != el) continue;
-            const int64_t recs = static_cast<int64_t>(s->fem->E) * s->fem->Q;
-            if (recs <= 0) continue;
-            if (recs > INT32_MAX) throw std::invalid_argument("scene too large for the gradient cache");
+            const int E = s->fem->E;
+            if (E <= 0) continue;
+            if (static_cast<int64_t>(E) * s->fem->Q > INT32_MAX)
+                throw std::invalid_argument("scene too large for the gradient cache");
             jl.S = s->fem->S;
             jl.Q = s->fem->Q;
             any = true;
-            const int nb = static_cast<int>((recs + kBlock - 1) / kBlock);
+            const int epb = refresh_elements_per_block(jl.S, jl.Q);
+            if (epb <= 0) throw std::invalid_argument("unsupported element");
+            const int nb = (E + epb - 1) / epb;
             const int sid = static_cast<int>(jsegs.size());
-            jsegs.push_back(VSeg{table_index[i], static_cast<int>(recs), static_cast<int>(jblock_seg.size())});
+            jsegs.push_back(VSeg{table_index[i], E, static_cast<int>(jblock_seg.size())});
             jblock_seg.insert(jblock_seg.end(), nb, sid);
         }
         jl.nblocks = static_cast<int>(jblock_seg.size()) - jl.block_base;
@@ -188,6 +216,8 @@ void DeviceSchedule::upload(Engine* eng, const Schedule& s, hipStream_t stream, 
     upload_grow(eng, vblock_seg, s.vblock_seg, stream);
     upload_grow(eng, jsegs, s.jsegs, stream);
     upload_grow(eng, jblock_seg, s.jblock_seg, stream);
+    upload_grow(eng, tsegs, s.tsegs, stream);
+    upload_grow(eng, tblock_seg, s.tblock_seg, stream);
     if (sync) OE_HIP(hipStreamSynchronize(stream));
 }
 
@@ -231,11 +261,13 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
     const int vblocks = static_cast<int>(sch.vblock_seg.size());
     BatchGatherArgs g{d_scenes, dev.segs.get(), dev.block_seg.get(), 0, 0, h};
     BatchRefreshArgs r{d_scenes, dev.jsegs.get(), dev.jblock_seg.get(), 0, 0};
+    BatchQ2TaskArgs tk{d_scenes, dev.tsegs.get(), dev.tblock_seg.get(), 0, 0, sch.q2_basis};
     for (int sub = 0; sub < substeps; ++sub) {
         launch_batch_predict(v, vblocks, stream);
         for (int it = 0; it < sch.max_iterations; ++it) {
             g.iteration = it;
             r.iteration = it;
+            tk.iteration = it;
             for (const Schedule::JLaunch& jl : sch.jlaunches) {
                 r.block_base = jl.block_base;
                 if (!launch_batch_ju_refresh(jl.S, jl.Q, r, jl.nblocks, stream))
@@ -244,8 +276,14 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
             for (const Schedule::GLaunch& gl : sch.glaunches) {
                 g.block_base = gl.block_base;
                 if (timer) timer->before(stream);
-                if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.cache, g, gl.nblocks, stream))
+                if (gl.mode == PASS_TASKS) {
+                    tk.block_base = gl.tblock_base;
+                    launch_batch_q2_tasks(tk, gl.tnblocks, stream);
+                    launch_batch_vertex_solve(g, gl.nblocks, stream);
+                } else if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.mode == PASS_CACHE, g, gl.nblocks,
+                                                stream)) {
                     throw std::invalid_argument("unsupported gather tile in batch");
+                }
                 if (timer) timer->after(stream);
             }
             v.iteration = it;

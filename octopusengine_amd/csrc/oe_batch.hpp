@@ -31,8 +31,9 @@ namespace oe {
 struct Schedule {
     struct GLaunch {
         int color, S, Q, L, QL;
-        bool cache;
-        int block_base, nblocks;
+        int mode;                   // PassMode
+        int block_base, nblocks;    // gather, or vertex solve (PASS_TASKS)
+        int tblock_base, tnblocks;  // element tasks (PASS_TASKS)
         int64_t slots;   // vertex slots covered (all scenes)
         int64_t incidences;
     };
@@ -47,8 +48,11 @@ struct Schedule {
     std::vector<VSeg> vsegs;
     std::vector<int> vblock_seg;
     std::vector<JLaunch> jlaunches;
-    std::vector<VSeg> jsegs;          // count = E * Q records of the scene
+    std::vector<VSeg> jsegs;          // count = the scene's elements
     std::vector<int> jblock_seg;
+    std::vector<BatchSeg> tsegs;      // element tasks: owners ranges
+    std::vector<int> tblock_seg;
+    q2::Basis1D q2_basis{};
     int max_iterations = 0;
     int64_t vertex_updates_per_substep = 0;
     double algorithmic_bytes_per_substep = 0.0;   // gather kernels, all iterations
@@ -67,6 +71,8 @@ struct DeviceSchedule {
     DBuf<int> vblock_seg;
     DBuf<VSeg> jsegs;
     DBuf<int> jblock_seg;
+    DBuf<BatchSeg> tsegs;
+    DBuf<int> tblock_seg;
     void upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync);
 };
 

@@ -228,40 +228,49 @@ oe_status oe_scene_set_graph(oe_scene* s, int32_t enable) {
 }
 
 namespace {
-void check_tile(int Q, int32_t lanes, int32_t qpl, int32_t cache) {
+void check_tile(int element, int32_t lanes, int32_t qpl, int32_t mode) {
+    const int Q = oe::elem_nb_quadrature(static_cast<oe::Element>(element));
     OE_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "lanes must be 4..64, power of 2");
     OE_REQUIRE(qpl >= 1 && Q % qpl == 0, "quadrature points per lane must divide Q");
     OE_REQUIRE(qpl == 1 || qpl == Q || (Q == 4 && qpl == 2) || (Q == 27 && (qpl == 3 || qpl == 9)),
                "tile not instantiated");
-    OE_REQUIRE(cache == 0 || cache == 1, "cache must be 0 or 1");
-    OE_REQUIRE(cache == 0 || qpl == 1, "the gradient cache takes one quadrature point per lane");
+    OE_REQUIRE(mode == oe::PASS_GATHER || mode == oe::PASS_CACHE || mode == oe::PASS_TASKS,
+               "mode must be 0 (gather), 1 (cached gradients) or 2 (element tasks)");
+    OE_REQUIRE(mode != oe::PASS_CACHE || qpl == 1, "the gradient cache takes one quadrature point per lane");
+    OE_REQUIRE(mode != oe::PASS_TASKS || element == oe::Hexa27, "the element-task pass is for 27-node hexahedra");
 }
 }  // namespace
 
-oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t cache) {
+oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mode) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
         oe::Scene& sc = *s->scene;
-        check_tile(sc.fem->Q, lanes, qpl, cache);
-        OE_REQUIRE(cache == 0 || sc.vbd_fem, "the gradient cache is a VBD option");
+        check_tile(sc.mesh.element, lanes, qpl, mode);
+        OE_REQUIRE(mode == oe::PASS_GATHER || sc.vbd_fem, "cached gradients and element tasks are VBD options");
+        OE_REQUIRE(mode != oe::PASS_TASKS || sc.vbd_fem->has_q2_basis, "element tasks unavailable for this element");
         sc.set_graph(false);
         if (sc.vbd_fem) {
-            sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, cache != 0};
+            sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, mode};
             if (sc.vbd_fem->cached()) sc.vbd_fem->ensure_ju();
+            if (sc.vbd_fem->tasks()) sc.vbd_fem->ensure_partial();
         }
-        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, false};
+        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, oe::PASS_GATHER};
     });
 }
 
-oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* cache) {
+oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* mode) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
-        oe::GatherTile t{0, 0, false};
+        oe::GatherTile t{0, 0, oe::PASS_GATHER};
         if (s->scene->vbd_fem) t = s->scene->vbd_fem->tile;
         if (s->scene->explicit_fem) t = s->scene->explicit_fem->tile;
         if (lanes) *lanes = t.L;
         if (qpl) *qpl = t.QL;
-        if (cache) *cache = (s->scene->vbd_fem && s->scene->vbd_fem->cached()) ? 1 : 0;
+        if (mode) {
+            *mode = oe::PASS_GATHER;
+            if (s->scene->vbd_fem && s->scene->vbd_fem->cached()) *mode = oe::PASS_CACHE;
+            if (s->scene->vbd_fem && s->scene->vbd_fem->tasks()) *mode = oe::PASS_TASKS;
+        }
     });
 }
 
@@ -402,11 +411,11 @@ oe_status oe_batch_set_graph(oe_batch* b, int32_t enable) {
     });
 }
 
-oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t cache) {
+oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t mode) {
     return guard([&] {
         OE_REQUIRE(b && oe::element_supported(element), "bad argument");
-        check_tile(oe::elem_nb_quadrature(static_cast<oe::Element>(element)), lanes, qpl, cache);
-        b->batch->set_tile(static_cast<oe::Element>(element), oe::GatherTile{lanes, qpl, cache != 0});
+        check_tile(element, lanes, qpl, mode);
+        b->batch->set_tile(static_cast<oe::Element>(element), oe::GatherTile{lanes, qpl, mode});
     });
 }
 

@@ -145,6 +145,50 @@ bool build_affine_records(const HostMesh& m, std::vector<float>& rec, std::vecto
     return true;
 }
 
+bool q2_basis_1d(q2::Basis1D& out) {
+    const ShapeTable& st = shape_table(Hexa27);
+    if (st.S != 27 || st.Q != 27) return false;
+    for (int k = 0; k < 27; ++k) {
+        const int t = q2::kNodeABC[k];
+        const int abc[3] = {t / 9, t / 3 % 3, t % 3};
+        for (int d = 0; d < 3; ++d)
+            if (st.nodes[3 * k + d] != static_cast<double>(abc[d] - 1)) return false;
+    }
+    // Gauss coordinates along x from the points (i, 1, 1); the same along y, z
+    double x[3];
+    for (int i = 0; i < 3; ++i) x[i] = st.qpoints[3 * q2::tensor_to_node(i * 9 + 4)];
+    double L[3][3], D[3][3];
+    for (int i = 0; i < 3; ++i) {
+        const double v = x[i];
+        L[0][i] = 0.5 * v * (v - 1.0);
+        L[1][i] = 1.0 - v * v;
+        L[2][i] = 0.5 * v * (v + 1.0);
+        D[0][i] = v - 0.5;
+        D[1][i] = -2.0 * v;
+        D[2][i] = v + 0.5;
+    }
+    for (int q = 0; q < 27; ++q) {
+        const int tq = q2::kNodeABC[q];
+        const int ijk[3] = {tq / 9, tq / 3 % 3, tq % 3};
+        for (int d = 0; d < 3; ++d)
+            if (st.qpoints[3 * q + d] != x[ijk[d]]) return false;
+        for (int k = 0; k < 27; ++k) {
+            const int t = q2::kNodeABC[k];
+            const int a = t / 9, b = t / 3 % 3, c = t % 3;
+            const double g[3] = {D[a][ijk[0]] * L[b][ijk[1]] * L[c][ijk[2]], L[a][ijk[0]] * D[b][ijk[1]] * L[c][ijk[2]],
+                                 L[a][ijk[0]] * L[b][ijk[1]] * D[c][ijk[2]]};
+            for (int d = 0; d < 3; ++d)
+                if (std::fabs(g[d] - st.dN[(q * 27 + k) * 3 + d]) > 1e-12) return false;
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < 3; ++i) {
+            out.L[a][i] = static_cast<float>(L[a][i]);
+            out.D[a][i] = static_cast<float>(D[a][i]);
+        }
+    return true;
+}
+
 std::vector<double> element_volumes(const HostMesh& m) {
     const ShapeTable& st = shape_table(m.element);
     const int S = st.S, Q = st.Q, E = m.nb_elements();

@@ -30,35 +30,81 @@
 
 namespace oe {
 
+// Work items of a lane whose gradient records stay in registers between the
+// gather and the update in the cached form: enough for vertices with up to 4
+// incident elements (ceil(4 Q / L), at most 2); further items are re-read.
+template <int Q, int L>
+constexpr int kept_items() {
+    return (4 * Q + L - 1) / L < 2 ? (4 * Q + L - 1) / L : 2;
+}
+
+// (e, q, local node) of work item t of a vertex whose owner records start at own
+__device__ __forceinline__ void work_item(const int* __restrict__ own, int t, int QC, int& eid, int& qc, int& r) {
+    const int j = t / QC;
+    qc = t - j * QC;
+    const int pk = own[j];
+    eid = pk >> 5;
+    r = pk & 31;
+}
+
+// column-major: Ju[c*3 + k] += du[k] dN_r[c]
+__device__ __forceinline__ void rank1_update(float* g, const float Ju[9], const float du[3],
+                                             const float* __restrict__ dn) {
+    const float d0 = dn[0], d1 = dn[1], d2 = dn[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        g[k] = fmaf(du[k], d0, Ju[k]);
+        g[3 + k] = fmaf(du[k], d1, Ju[3 + k]);
+        g[6 + k] = fmaf(du[k], d2, Ju[6 + k]);
+    }
+}
+
 // Gather + local solve of one vertex slot by a group of L lanes.
 template <int S, int Q, int L, int QL, bool CACHED>
 __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int lane) {
     static_assert(!CACHED || QL == 1, "cached gradients take one quadrature point per work item");
     constexpr int QC = Q / QL;
+    constexpr int KEEP = CACHED ? kept_items<Q, L>() : 1;
     if (grp >= a.count) return;                 // whole group leaves together
     const int slot = a.first_slot + grp;
     const int vid = a.own.slot_vid[slot];
     if (a.mode != GATHER_EXPLICIT_FORCE && a.ps.mask[vid] == 0) return;
     const int2 nbo = a.own.slot_nbo[slot];
     const int nwork = nbo.x * QC;
+    const int* __restrict__ own = a.own.own + nbo.y;
     Acc9 acc;
     acc_zero(acc);
     const float4* __restrict__ P = a.ps.p;
-    for (int t = lane; t < nwork; t += L) {
-        const int j = t / QC;
-        const int qc = t - j * QC;
-        const int pk = a.own.own[nbo.y + j];
-        const int eid = pk >> 5;
-        const int r = pk & 31;
-        if constexpr (CACHED) {
-            const int q = qc;
+    float keep[KEEP][9];
+    if constexpr (CACHED) {
+#pragma unroll
+        for (int it = 0; it < KEEP; ++it) {
+            const int t = lane + it * L;
+            if (t < nwork) {
+                int eid, q, r;
+                work_item(own, t, Q, eid, q, r);
+                const float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
+                float JXi[9], V;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) keep[it][i] = g[i];
+                load_element_record(a.fem, eid, q, JXi, V);
+                nh_contrib(keep[it], JXi, V, a.fem.dN + (q * S + r) * 3, a.mt, acc);
+            }
+        }
+        for (int t = lane + KEEP * L; t < nwork; t += L) {
+            int eid, q, r;
+            work_item(own, t, Q, eid, q, r);
             const float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
             float Ju[9], JXi[9], V;
 #pragma unroll
             for (int i = 0; i < 9; ++i) Ju[i] = g[i];
             load_element_record(a.fem, eid, q, JXi, V);
             nh_contrib(Ju, JXi, V, a.fem.dN + (q * S + r) * 3, a.mt, acc);
-        } else {
+        }
+    } else {
+        for (int t = lane; t < nwork; t += L) {
+            int eid, qc, r;
+            work_item(own, t, QC, eid, qc, r);
             const int* __restrict__ tp = a.fem.topo + static_cast<size_t>(eid) * S;
             float xs[S][3];
 #pragma unroll
@@ -117,22 +163,23 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
 #pragma unroll
         for (int k = 0; k < 3; ++k) du[k] = __shfl(du[k], 0, L);
         if (du[0] == 0.f && du[1] == 0.f && du[2] == 0.f) return;
-        for (int t = lane; t < nwork; t += L) {
-            const int j = t / Q;
-            const int q = t - j * Q;
-            const int pk = a.own.own[nbo.y + j];
-            const int eid = pk >> 5;
-            const int r = pk & 31;
-            const float* __restrict__ dn = a.fem.dN + (q * S + r) * 3;
-            const float d0 = dn[0], d1 = dn[1], d2 = dn[2];
-            float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
-            // column-major: Ju[c*3 + k] += du[k] dN_r[c]
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                g[k] = fmaf(du[k], d0, g[k]);
-                g[3 + k] = fmaf(du[k], d1, g[3 + k]);
-                g[6 + k] = fmaf(du[k], d2, g[6 + k]);
+        for (int it = 0; it < KEEP; ++it) {
+            const int t = lane + it * L;
+            if (t < nwork) {
+                int eid, q, r;
+                work_item(own, t, Q, eid, q, r);
+                rank1_update(a.ju + (static_cast<size_t>(eid) * Q + q) * 9, keep[it], du, a.fem.dN + (q * S + r) * 3);
             }
+        }
+        for (int t = lane + KEEP * L; t < nwork; t += L) {
+            int eid, q, r;
+            work_item(own, t, Q, eid, q, r);
+            float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
+            float Ju[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Ju[i] = g[i];
+            rank1_update(g, Ju, du, a.fem.dN + (q * S + r) * 3);
         }
     }
 }
@@ -175,41 +222,63 @@ __global__ __launch_bounds__(256, gather_min_waves<QL>()) void k_batch_gather(Ba
     gather_body<S, Q, L, QL, CACHED>(a, local / L, local & (L - 1));
 }
 
-// Ju(e, q) = sum_k u_k (x) dN_k(q) for record idx = e * Q + q.
+// Refresh of the cached gradients, Ju(e, q) = sum_k u_k (x) dN_k(q): a block
+// takes REFRESH_EPB(Q) = 256 / Q consecutive elements, stages their S node
+// displacements in LDS once (one 16-byte load per node instead of one per
+// (node, quadrature point)), then thread (element, q) builds its record; the
+// records of one element are contiguous, so the stores are coalesced.
+template <int Q>
+constexpr int refresh_epb() { return 256 / Q; }
+
 template <int S, int Q>
-__device__ __forceinline__ void ju_refresh_one(const float4* __restrict__ P, const FemView& f, float* ju, int idx) {
-    const int e = idx / Q;
-    const int q = idx - e * Q;
-    const int* __restrict__ tp = f.topo + static_cast<size_t>(e) * S;
+__device__ __forceinline__ void ju_refresh_block(const float4* __restrict__ P, const FemView& f, float* ju, int e0,
+                                                 int ne) {
+    constexpr int EPB = refresh_epb<Q>();
+    __shared__ float sx[EPB * S * 3];
+    for (int t = threadIdx.x; t < ne * S; t += blockDim.x) {
+        const float4 x = P[f.topo[static_cast<size_t>(e0) * S + t]];
+        sx[3 * t] = x.x;
+        sx[3 * t + 1] = x.y;
+        sx[3 * t + 2] = x.z;
+    }
+    __syncthreads();
+    const int le = threadIdx.x / Q;
+    const int q = threadIdx.x - le * Q;
+    if (le >= ne) return;
     float xs[S][3];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const float4 x = P[tp[k]];
-        xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+        xs[k][0] = sx[(le * S + k) * 3];
+        xs[k][1] = sx[(le * S + k) * 3 + 1];
+        xs[k][2] = sx[(le * S + k) * 3 + 2];
     }
     float Ju[9];
     jx_from_nodes<S>(xs, f.dN + q * S * 3, Ju);
-    float* g = ju + static_cast<size_t>(idx) * 9;
+    float* g = ju + (static_cast<size_t>(e0 + le) * Q + q) * 9;
 #pragma unroll
     for (int i = 0; i < 9; ++i) g[i] = Ju[i];
 }
 
 template <int S, int Q>
 __global__ __launch_bounds__(256) void k_ju_refresh(PSView ps, FemView f, float* ju) {
-    const int idx = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (idx >= f.E * Q) return;
-    ju_refresh_one<S, Q>(ps.p, f, ju, idx);
+    constexpr int EPB = refresh_epb<Q>();
+    const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
+    const int ne = min(EPB, f.E - e0);
+    ju_refresh_block<S, Q>(ps.p, f, ju, e0, ne);
 }
 
+// Batched form: VSeg.count is the scene's element count, each block takes
+// REFRESH_EPB elements of one scene.
 template <int S, int Q>
 __global__ __launch_bounds__(256) void k_batch_ju_refresh(BatchRefreshArgs b) {
+    constexpr int EPB = refresh_epb<Q>();
     const int blk = b.block_base + xcd_block(blockIdx.x, gridDim.x);
     const VSeg seg = b.segs[b.block_seg[blk]];
     const SceneDev& sd = b.scenes[seg.scene];
-    if (b.iteration >= sd.iterations) return;
-    const int idx = (blk - seg.block_begin) * 256 + threadIdx.x;
-    if (idx >= seg.count) return;
-    ju_refresh_one<S, Q>(sd.ps.p, sd.fem, sd.ju, idx);
+    if (b.iteration >= sd.iterations) return;      // uniform over the block
+    const int e0 = (blk - seg.block_begin) * EPB;
+    const int ne = min(EPB, seg.count - e0);
+    ju_refresh_block<S, Q>(sd.ps.p, sd.fem, sd.ju, e0, ne);
 }
 
 namespace {
@@ -295,10 +364,12 @@ bool OE_FN(launch_batch_gather_)(int L, int QL, bool cached, const BatchGatherAr
 
 void OE_FN(launch_ju_refresh_)(const PSView& ps, const FemView& f, float* ju, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
-    const long long n = static_cast<long long>(f.E) * Q;
-    if (n <= 0) return;
-    hipLaunchKernelGGL((k_ju_refresh<S, Q>), dim3(static_cast<int>((n + 255) / 256)), dim3(256), 0, s, ps, f, ju);
+    constexpr int EPB = refresh_epb<Q>();
+    if (f.E <= 0) return;
+    hipLaunchKernelGGL((k_ju_refresh<S, Q>), dim3((f.E + EPB - 1) / EPB), dim3(256), 0, s, ps, f, ju);
 }
+
+int OE_FN(refresh_elements_per_block_)() { return refresh_epb<OE_GATHER_Q>(); }
 
 void OE_FN(launch_batch_ju_refresh_)(const BatchRefreshArgs& b, int nblocks, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;

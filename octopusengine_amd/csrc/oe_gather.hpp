@@ -44,7 +44,8 @@ struct SceneDev {
     OwnersView own;
     MaterialView mt;
     float4* y;
-    float* ju;   // cached displacement gradients, or null
+    float* ju;        // cached displacement gradients, or null
+    float* partial;   // element-task partials (9 floats per owners entry), or null
     float damping;
     int iterations;
     float g[3];
@@ -77,7 +78,8 @@ struct BatchGatherArgs {
 };
 
 // Batched refresh of the cached displacement gradients: VSeg.count is the
-// scene's E * Q records.
+// scene's element count; each block takes refresh_elements_per_block(S, Q)
+// elements.
 struct BatchRefreshArgs {
     const SceneDev* scenes;
     const VSeg* segs;
@@ -91,7 +93,8 @@ struct BatchRefreshArgs {
     bool launch_batch_gather_##S##_##Q(int L, int QL, bool cached, const BatchGatherArgs& b, int nblocks,     \
                                        hipStream_t s);                                                        \
     void launch_ju_refresh_##S##_##Q(const PSView& ps, const FemView& f, float* ju, hipStream_t s);          \
-    void launch_batch_ju_refresh_##S##_##Q(const BatchRefreshArgs& b, int nblocks, hipStream_t s);
+    void launch_batch_ju_refresh_##S##_##Q(const BatchRefreshArgs& b, int nblocks, hipStream_t s);          \
+    int refresh_elements_per_block_##S##_##Q();
 OE_GATHER_DECL(4, 1)
 OE_GATHER_DECL(8, 8)
 OE_GATHER_DECL(10, 4)
@@ -117,6 +120,15 @@ inline bool launch_ju_refresh(int S, int Q, const PSView& ps, const FemView& f, 
     else if (S == 27 && Q == 27) launch_ju_refresh_27_27(ps, f, ju, s);
     else return false;
     return true;
+}
+
+inline int refresh_elements_per_block(int S, int Q) {
+    if (S == 4 && Q == 1) return refresh_elements_per_block_4_1();
+    if (S == 8 && Q == 8) return refresh_elements_per_block_8_8();
+    if (S == 10 && Q == 4) return refresh_elements_per_block_10_4();
+    if (S == 20 && Q == 11) return refresh_elements_per_block_20_11();
+    if (S == 27 && Q == 27) return refresh_elements_per_block_27_27();
+    return 0;
 }
 
 inline bool launch_batch_ju_refresh(int S, int Q, const BatchRefreshArgs& b, int nblocks, hipStream_t s) {

@@ -17,6 +17,7 @@
 
 #include "oe_diag.hpp"
 #include "oe_gather.hpp"
+#include "oe_q2.hpp"
 #include "oe_xpbd.hpp"
 
 namespace oe {
@@ -235,12 +236,12 @@ FemView GPU_FEM::fem_view() const {
 
 GatherTile default_tile(Element e) {
     switch (e) {
-        case Tetra: return {8, 1, false};
-        case Hexa: return {16, 1, false};
-        case Tetra10: return {16, 2, false};
-        case Tetra20: return {64, 1, false};
-        case Hexa27: return {64, 1, true};
-        default: return {16, 1, false};
+        case Tetra: return {8, 1, PASS_GATHER};
+        case Hexa: return {16, 1, PASS_GATHER};
+        case Tetra10: return {16, 2, PASS_GATHER};
+        case Tetra20: return {64, 1, PASS_GATHER};
+        case Hexa27: return {64, 1, PASS_TASKS};
+        default: return {16, 1, PASS_GATHER};
     }
 }
 
@@ -269,12 +270,19 @@ GPU_VBD_FEM::GPU_VBD_FEM(Engine* e, Element el, const HostMesh& mesh, Material m
     owners = build_owners(mesh, colors);
     upload_owners(eng, owners, d_slot_vid, d_slot_nbo, d_own);
     if (version == Better_Coloration) p_new.allocate(&eng->pool, mesh.nb_vertices());
+    if (el == Hexa27) has_q2_basis = q2_basis_1d(q2_basis);
     // allocated here, not in step(), which may run under HIP graph capture
     if (cached()) ensure_ju();
+    if (tasks()) ensure_partial();
 }
 
 GPU_VBD_FEM::~GPU_VBD_FEM() {
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
+}
+
+float* GPU_VBD_FEM::ensure_partial() {
+    if (!partial.get()) partial.allocate(&eng->pool, owners.own.size() * 9);
+    return partial.get();
 }
 
 float* GPU_VBD_FEM::ensure_ju() {
@@ -345,6 +353,8 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
         a.ju = ju.get();
         if (!launch_ju_refresh(S, Q, a.ps, a.fem, a.ju, s)) throw std::invalid_argument("unsupported element");
     }
+    const bool by_tasks = tasks();
+    if (by_tasks && !partial.get()) throw std::logic_error("task partials not allocated");
     for (int c : order) {
         a.first_slot = owners.color_begin[c];
         a.count = owners.color_begin[c + 1] - a.first_slot;
@@ -356,7 +366,15 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
             }
             OE_HIP(hipEventRecord(ev_pool[ev_used], s));
         }
-        if (!launch_gather(S, Q, tile.L, tile.QL, a, s)) throw std::invalid_argument("unsupported gather tile shape");
+        if (by_tasks) {
+            const int first_own = a.count > 0 ? owners.slot_off[a.first_slot] : 0;
+            launch_q2_tasks(Q2TaskArgs{a.ps, a.fem, a.mt, q2_basis, d_own.get(), partial.get(), first_own,
+                                       static_cast<int>(owners.color_incidences[c])},
+                            s);
+            launch_vertex_solve(VertexSolveArgs{a.ps, a.own, y, partial.get(), damping, dt, a.first_slot, a.count}, s);
+        } else if (!launch_gather(S, Q, tile.L, tile.QL, a, s)) {
+            throw std::invalid_argument("unsupported gather tile shape");
+        }
         if (time_gather) {
             OE_HIP(hipEventRecord(ev_pool[ev_used + 1], s));
             ev_used += 2;

@@ -125,13 +125,17 @@ struct GPU_FEM : GPU_Dynamic {
 
 enum VBD_Version : int { Base = 0, Threaded_Quadrature = 1, Reduction_Symmetry = 2, Better_Coloration = 3 };
 
-// Tile shape of the per-vertex gather: lanes per vertex, quadrature points per
-// lane, and whether the VBD sweep keeps the displacement gradients per
-// (element, quadrature point) between colours (QL = 1 only).
+// How a VBD colour pass runs, per element type:
+//   gather  one kernel, L lanes per vertex, QL quadrature points per lane;
+//   cache   the same with the displacement gradients of every (element,
+//           quadrature point) kept in device memory between colours (QL = 1);
+//   tasks   27-node hexahedra only: element tasks with sum-factorised
+//           gradients, then a vertex-solve kernel (L, QL unused).
+enum PassMode : int { PASS_GATHER = 0, PASS_CACHE = 1, PASS_TASKS = 2 };
 struct GatherTile {
     int L = 16;
     int QL = 1;
-    bool cache = false;
+    int mode = PASS_GATHER;
 };
 GatherTile default_tile(Element e);
 
@@ -162,7 +166,13 @@ struct GPU_VBD_FEM final : GPU_FEM {
     // positions (3N floats; fixed vertices keep 0; synchronising).
     void residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r);
     DBuf<float4> d_residual;
-    bool cached() const { return tile.cache && version != Better_Coloration; }
+    bool cached() const { return tile.mode == PASS_CACHE && version != Better_Coloration; }
+    // element-task partials, 9 floats per owners entry (allocated on first use)
+    DBuf<float> partial;
+    float* ensure_partial();
+    bool tasks() const { return tile.mode == PASS_TASKS && has_q2_basis && version != Better_Coloration; }
+    bool has_q2_basis = false;
+    q2::Basis1D q2_basis{};
     int64_t conflicts = 0;
     // optional per-launch timing of the gather kernel (events around each colour)
     bool time_gather = false;

@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include "oe_q2_tensor.hpp"
+
 namespace oe {
 
 // Element ids keep the integer values of the reference enum
@@ -144,6 +146,11 @@ struct ElementSchedule {
     std::vector<int32_t> color_begin;   // nb_color + 1
 };
 ElementSchedule build_element_schedule(const Coloration& col, int nb_elements);
+
+// 1D factors of the 27-node hexahedron's shape functions at the Gauss
+// coordinates (see oe_q2_tensor.hpp); false if the element's shape table does
+// not factor with the node and point orders of that header.
+bool q2_basis_1d(q2::Basis1D& out);
 
 // Mesh files (reference VTK_Loader / Msh_Loader,
 // src/Mesh/Converter/MeshLoader.cpp:27-237): float coordinates and one

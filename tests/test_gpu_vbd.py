@@ -113,10 +113,50 @@ def test_vbd_cached_gradients(engine, case):
         assert e_orc <= TOL1
 
 
+@pytest.mark.parametrize("curve", [0.0, 0.02])
+def test_vbd_q2_element_tasks(engine, curve):
+    """Q2 colour pass as element tasks (sum-factorised gradients) + vertex
+    solves: same steps as the gather, and the oracle's, on straight (per-element
+    records) and curved (per-point records) meshes."""
+    import octopusengine_amd as oe
+
+    res = {}
+    for mode in (oe.PASS_GATHER, oe.PASS_TASKS):
+        g, o, _, _ = make_pair(engine, 9, (4, 2, 2), curve=curve, tile=(64, 1, mode))
+        assert g.get_tile()[2] == mode
+        assert g.affine() == (curve == 0.0)
+        for _ in range(2):
+            g.step(0.005, 1)
+            o.step(0.005, 1)
+        res[mode] = g.positions()
+        err = max_err(res[mode], o.positions())
+        print(f"Q2 curve={curve} mode={mode} 2-step err={err:.3e}", flush=True)
+        assert err <= TOL1
+    pair = max_err(res[oe.PASS_GATHER], res[oe.PASS_TASKS])
+    print(f"Q2 tasks vs gather {pair:.3e}", flush=True)
+    assert pair <= TOL1 / 10
+
+
+def test_vbd_q2_element_tasks_many_steps(engine):
+    import octopusengine_amd as oe
+
+    g, o, rest, _ = make_pair(engine, 9, (4, 2, 2), tile=(64, 1, oe.PASS_TASKS))
+    g.set_graph(True)
+    for _ in range(40):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    err = max_err(g.positions(), o.positions())
+    disp = float(np.abs(o.positions() - rest).max())
+    print(f"Q2 tasks 40 steps (graph) err={err:.3e} displacement={disp:.3e}", flush=True)
+    assert disp > 1e-3 and err <= TOL100
+
+
 def test_gradient_cache_tile_checks(engine):
     import octopusengine_amd as oe
 
     g, _, _, _ = make_pair(engine, 7, (2, 1, 1))
+    with pytest.raises(oe.OEError):
+        g.set_tile(16, 1, oe.PASS_TASKS)  # element tasks are for 27-node hexahedra
     with pytest.raises(oe.OEError):
         g.set_tile(16, 2, True)          # the cache takes one quadrature point per lane
     x, _, _, _ = make_pair(engine, 7, (2, 1, 1), solver=oe.SOLVER_EXPLICIT, iterations=1)
