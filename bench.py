@@ -167,7 +167,7 @@ def batched_leg(args, eng) -> dict:
     bytes_total = info["gather_bytes"] * args.steps
     avg_launch_s = gather_ms / 1e3 / max(launches, 1)
     achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
-    traffic = measured_traffic("k_batch_gather", f"batched{args.scenes}_seed{args.seed}")
+    traffic = measured_traffic("vbd_colour_pass", f"batched{args.scenes}_seed{args.seed}")
     b.close()
     return {
         "value": vu_step * args.steps / wall,
@@ -185,7 +185,8 @@ def batched_leg(args, eng) -> dict:
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_detail": traffic,
-                     "kernel": "k_batch_gather (VBD colour pass, all scenes)",
+                     "kernel": "VBD colour pass over all scenes: k_batch_gather (P2), "
+                               "k_batch_q2_tasks + k_batch_vertex_solve (Q2)",
                      "bytes_per_launch": bytes_total / max(launches, 1), "avg_launch_us": avg_launch_s * 1e6,
                      "launches": launches},
         "_scenes": scenes,
