@@ -396,7 +396,7 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=1024, help="resident scenes in the served pool")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson request stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="step requests per second")
-    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-max-batch", type=int, default=16)
     ap.add_argument("--serve-streams", type=int, default=2)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--eager-only", action="store_true",
@@ -436,14 +436,21 @@ def main():
         "roofline": roof,
         "p50_substep_ms": bat["p50_substep_ms"],
         "p99_substep_ms": bat["p99_substep_ms"],
+        "latency_source": "batched step of all 256 scenes (graph replay)",
         "batched": {k: bat[k] for k in ("graph_value", "graph_ms_per_step", "plain_value", "plain_ms_per_step",
-                                        "speedup_vs_plain", "setup_s", "positions_finite")},
+                                        "speedup_vs_plain", "p50_substep_ms", "p99_substep_ms", "setup_s",
+                                        "positions_finite")},
     }
     for sc in scenes:
         sc.close()
     del scenes
     if not args.no_serve:
         out["served"] = served_leg(args)
+        # the substep-latency target is a serving figure: request (one step)
+        # latency under the Poisson load
+        out["p50_substep_ms"] = out["served"]["p50_ms"]
+        out["p99_substep_ms"] = out["served"]["p99_ms"]
+        out["latency_source"] = "served: Poisson step requests, latency = completion - submission"
     if not args.no_single:
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
     if args.copy_bytes > 0:
