@@ -396,7 +396,7 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=1024, help="resident scenes in the served pool")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson request stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="step requests per second")
-    ap.add_argument("--serve-max-batch", type=int, default=16)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--serve-streams", type=int, default=2)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--eager-only", action="store_true",
@@ -444,6 +444,10 @@ def main():
     for sc in scenes:
         sc.close()
     del scenes
+    # one engine (one stream) at a time: the server adds its own streams, and a
+    # process gets 4 hardware queues, beyond which streams share a queue and
+    # serialise
+    eng.close()
     if not args.no_serve:
         out["served"] = served_leg(args)
         # the substep-latency target is a serving figure: request (one step)
@@ -452,7 +456,9 @@ def main():
         out["p99_substep_ms"] = out["served"]["p99_ms"]
         out["latency_source"] = "served: Poisson step requests, latency = completion - submission"
     if not args.no_single:
+        eng = oe.Engine(0)
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
+        eng.close()
     if args.copy_bytes > 0:
         out["copy_calibration"] = copy_calibration(args.copy_bytes)
     if not args.no_cpu:

@@ -112,8 +112,8 @@ oe_status oe_scene_set_graph(oe_scene* s, int32_t enable);
  * quadrature points per lane of the gather kernel, and mode = 0 (gather),
  * 1 (gather with the displacement gradients of every (element, quadrature
  * point) kept in device memory during a sweep; qpl must be 1) or 2 (element
- * tasks with sum-factorised gradients then vertex solves; 27-node hexahedra
- * only).  Modes 1 and 2 are VBD options.  No counterpart in the reference,
+ * tasks - one per incidence of the colour, sum-factorised gradients for
+ * 27-node hexahedra - then vertex solves).  Modes 1 and 2 are VBD options.  No counterpart in the reference,
  * whose kernels fix the thread layout per version
  * (src/GPU/VBD/GPU_VBD_FEM.cu:712-756). */
 oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mode);

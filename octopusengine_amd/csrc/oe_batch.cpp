@@ -62,8 +62,8 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
     check_batchable(s);
     SceneDev d{};
     const GatherTile t = tile_of(tiles, s.mesh.element);
-    if (t.mode == PASS_TASKS && !s.vbd_fem->has_q2_basis)
-        throw std::invalid_argument("the element-task pass is for 27-node hexahedra");
+    if (t.mode == PASS_TASKS && s.mesh.element == Hexa27 && !s.vbd_fem->has_q2_basis)
+        throw std::invalid_argument("no tensor factors for this 27-node hexahedron");
     d.ju = t.mode == PASS_CACHE ? s.vbd_fem->ensure_ju() : nullptr;
     d.partial = t.mode == PASS_TASKS ? s.vbd_fem->ensure_partial() : nullptr;
     d.ps = s.ps->get_parameters();
@@ -94,15 +94,12 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
 
 void Schedule::clear() {
     segs.clear();
-    block_seg.clear();
     glaunches.clear();
     vsegs.clear();
-    vblock_seg.clear();
+    vblocks = 0;
     tsegs.clear();
-    tblock_seg.clear();
     jlaunches.clear();
     jsegs.clear();
-    jblock_seg.clear();
     max_iterations = 0;
     vertex_updates_per_substep = 0;
     algorithmic_bytes_per_substep = 0.0;
@@ -125,7 +122,8 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
         for (Element el : order) {
             GLaunch gl{};
             gl.color = c;
-            gl.block_base = static_cast<int>(block_seg.size());
+            gl.seg_first = static_cast<int>(segs.size());
+            gl.tseg_first = static_cast<int>(tsegs.size());
             bool any = false;
             for (size_t i = 0; i < scenes.size(); ++i) {
                 Scene* s = scenes[i];
@@ -140,44 +138,40 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     gl.L = t.L;
                     gl.QL = t.QL;
                     gl.mode = t.mode;
-                    gl.tblock_base = static_cast<int>(tblock_seg.size());
                     any = true;
                     if (gl.mode == PASS_TASKS) q2_basis = s->vbd_fem->q2_basis;
                 }
                 // gather: L lanes per slot; tasks: the vertex solve takes one thread per slot
                 const int lanes = gl.mode == PASS_TASKS ? 1 : gl.L;
                 const int nb = static_cast<int>((static_cast<int64_t>(count) * lanes + kBlock - 1) / kBlock);
-                const int sid = static_cast<int>(segs.size());
-                segs.push_back(BatchSeg{table_index[i], first, count, static_cast<int>(block_seg.size())});
-                block_seg.insert(block_seg.end(), nb, sid);
+                segs.push_back(BatchSeg{table_index[i], first, count, gl.nblocks});
+                gl.nblocks += nb;
                 if (gl.mode == PASS_TASKS) {
                     const int tasks = static_cast<int>(s->vbd_fem->owners.color_incidences[c]);
-                    const int tpb = q2_tasks_per_block();
-                    const int tnb = (tasks + tpb - 1) / tpb;
-                    const int tid = static_cast<int>(tsegs.size());
-                    tsegs.push_back(BatchSeg{table_index[i], s->vbd_fem->owners.slot_off[first], tasks,
-                                             static_cast<int>(tblock_seg.size())});
-                    tblock_seg.insert(tblock_seg.end(), tnb, tid);
+                    const int tpb = el == Hexa27 ? q2_tasks_per_block() : kElemTasksPerBlock;
+                    if (tasks > 0) {
+                        tsegs.push_back(BatchSeg{table_index[i], s->vbd_fem->owners.slot_off[first], tasks,
+                                                 gl.tnblocks});
+                        gl.tnblocks += (tasks + tpb - 1) / tpb;
+                    }
                 }
                 gl.slots += count;
                 gl.incidences += s->vbd_fem->owners.color_incidences[c];
             }
-            gl.nblocks = static_cast<int>(block_seg.size()) - gl.block_base;
-            gl.tnblocks = static_cast<int>(tblock_seg.size()) - gl.tblock_base;
+            gl.seg_count = static_cast<int>(segs.size()) - gl.seg_first;
+            gl.tseg_count = static_cast<int>(tsegs.size()) - gl.tseg_first;
             if (any) glaunches.push_back(gl);
         }
     }
     for (size_t i = 0; i < scenes.size(); ++i) {
         const int n = scenes[i]->ps->n;
-        const int nb = (n + kBlock - 1) / kBlock;
-        const int sid = static_cast<int>(vsegs.size());
-        vsegs.push_back(VSeg{table_index[i], n, static_cast<int>(vblock_seg.size())});
-        vblock_seg.insert(vblock_seg.end(), nb, sid);
+        vsegs.push_back(VSeg{table_index[i], n, vblocks});
+        vblocks += (n + kBlock - 1) / kBlock;
     }
     for (Element el : order) {
         if (tile_of(tiles, el).mode != PASS_CACHE) continue;
         JLaunch jl{};
-        jl.block_base = static_cast<int>(jblock_seg.size());
+        jl.seg_first = static_cast<int>(jsegs.size());
         bool any = false;
         for (size_t i = 0; i < scenes.size(); ++i) {
             Scene* s = scenes[i];
@@ -191,12 +185,10 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
             any = true;
             const int epb = refresh_elements_per_block(jl.S, jl.Q);
             if (epb <= 0) throw std::invalid_argument("unsupported element");
-            const int nb = (E + epb - 1) / epb;
-            const int sid = static_cast<int>(jsegs.size());
-            jsegs.push_back(VSeg{table_index[i], E, static_cast<int>(jblock_seg.size())});
-            jblock_seg.insert(jblock_seg.end(), nb, sid);
+            jsegs.push_back(VSeg{table_index[i], E, jl.nblocks});
+            jl.nblocks += (E + epb - 1) / epb;
         }
-        jl.nblocks = static_cast<int>(jblock_seg.size()) - jl.block_base;
+        jl.seg_count = static_cast<int>(jsegs.size()) - jl.seg_first;
         if (any) jlaunches.push_back(jl);
     }
 }
@@ -211,13 +203,9 @@ void upload_grow(Engine* eng, DBuf<T>& buf, const std::vector<T>& h, hipStream_t
 
 void DeviceSchedule::upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync) {
     upload_grow(eng, segs, s.segs, stream);
-    upload_grow(eng, block_seg, s.block_seg, stream);
     upload_grow(eng, vsegs, s.vsegs, stream);
-    upload_grow(eng, vblock_seg, s.vblock_seg, stream);
     upload_grow(eng, jsegs, s.jsegs, stream);
-    upload_grow(eng, jblock_seg, s.jblock_seg, stream);
     upload_grow(eng, tsegs, s.tsegs, stream);
-    upload_grow(eng, tblock_seg, s.tblock_seg, stream);
     if (sync) OE_HIP(hipStreamSynchronize(stream));
 }
 
@@ -257,28 +245,36 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                      hipStream_t stream, GatherTimer* timer) {
     if (sch.vsegs.empty()) return;
     const float h = dt / static_cast<float>(substeps);
-    BatchVArgs v{d_scenes, dev.vsegs.get(), dev.vblock_seg.get(), h, 0};
-    const int vblocks = static_cast<int>(sch.vblock_seg.size());
-    BatchGatherArgs g{d_scenes, dev.segs.get(), dev.block_seg.get(), 0, 0, h};
-    BatchRefreshArgs r{d_scenes, dev.jsegs.get(), dev.jblock_seg.get(), 0, 0};
-    BatchQ2TaskArgs tk{d_scenes, dev.tsegs.get(), dev.tblock_seg.get(), 0, 0, sch.q2_basis};
+    BatchVArgs v{d_scenes, dev.vsegs.get(), static_cast<int>(sch.vsegs.size()), h, 0};
+    BatchGatherArgs g{d_scenes, dev.segs.get(), 0, 0, 0, h};
+    BatchRefreshArgs r{d_scenes, dev.jsegs.get(), 0, 0, 0};
+    BatchQ2TaskArgs tk{d_scenes, dev.tsegs.get(), 0, 0, 0, sch.q2_basis};
+    BatchElemTaskArgs et{d_scenes, dev.tsegs.get(), 0, 0, 0};
     for (int sub = 0; sub < substeps; ++sub) {
-        launch_batch_predict(v, vblocks, stream);
+        launch_batch_predict(v, sch.vblocks, stream);
         for (int it = 0; it < sch.max_iterations; ++it) {
-            g.iteration = it;
-            r.iteration = it;
-            tk.iteration = it;
+            g.iteration = r.iteration = tk.iteration = et.iteration = it;
             for (const Schedule::JLaunch& jl : sch.jlaunches) {
-                r.block_base = jl.block_base;
+                r.seg_first = jl.seg_first;
+                r.seg_count = jl.seg_count;
                 if (!launch_batch_ju_refresh(jl.S, jl.Q, r, jl.nblocks, stream))
                     throw std::invalid_argument("unsupported element in batch");
             }
             for (const Schedule::GLaunch& gl : sch.glaunches) {
-                g.block_base = gl.block_base;
+                g.seg_first = gl.seg_first;
+                g.seg_count = gl.seg_count;
                 if (timer) timer->before(stream);
                 if (gl.mode == PASS_TASKS) {
-                    tk.block_base = gl.tblock_base;
-                    launch_batch_q2_tasks(tk, gl.tnblocks, stream);
+                    if (gl.S == 27 && gl.Q == 27) {
+                        tk.seg_first = gl.tseg_first;
+                        tk.seg_count = gl.tseg_count;
+                        if (gl.tseg_count > 0) launch_batch_q2_tasks(tk, gl.tnblocks, stream);
+                    } else if (gl.tseg_count > 0) {
+                        et.seg_first = gl.tseg_first;
+                        et.seg_count = gl.tseg_count;
+                        if (!launch_batch_elem_tasks(gl.S, gl.Q, et, gl.tnblocks, stream))
+                            throw std::invalid_argument("unsupported element in batch");
+                    }
                     launch_batch_vertex_solve(g, gl.nblocks, stream);
                 } else if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.mode == PASS_CACHE, g, gl.nblocks,
                                                 stream)) {
@@ -287,9 +283,9 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                 if (timer) timer->after(stream);
             }
             v.iteration = it;
-            launch_batch_fix_box(v, vblocks, stream);
+            launch_batch_fix_box(v, sch.vblocks, stream);
         }
-        launch_batch_velocity(v, vblocks, stream);
+        launch_batch_velocity(v, sch.vblocks, stream);
     }
 }
 

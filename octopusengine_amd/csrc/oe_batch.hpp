@@ -28,30 +28,30 @@
 namespace oe {
 
 // Launch plan for a set of scenes.  Scene indices refer to a SceneDev table.
+// Every launch covers a contiguous run of segments whose block_begin values
+// are launch-local; kernels find their segment by binary search, so the plan
+// is O(segments), not O(blocks), to build and upload.
 struct Schedule {
     struct GLaunch {
         int color, S, Q, L, QL;
-        int mode;                   // PassMode
-        int block_base, nblocks;    // gather, or vertex solve (PASS_TASKS)
-        int tblock_base, tnblocks;  // element tasks (PASS_TASKS)
+        int mode;                         // PassMode
+        int seg_first, seg_count, nblocks;      // gather, or vertex solve (PASS_TASKS)
+        int tseg_first, tseg_count, tnblocks;   // element tasks (PASS_TASKS)
         int64_t slots;   // vertex slots covered (all scenes)
         int64_t incidences;
     };
     // refresh of cached gradients: one launch per element type, over jsegs
     struct JLaunch {
         int S, Q;
-        int block_base, nblocks;
+        int seg_first, seg_count, nblocks;
     };
     std::vector<BatchSeg> segs;
-    std::vector<int> block_seg;
     std::vector<GLaunch> glaunches;   // ascending colour, then element type
-    std::vector<VSeg> vsegs;
-    std::vector<int> vblock_seg;
+    std::vector<VSeg> vsegs;          // every vertex of every scene, one launch
+    int vblocks = 0;
     std::vector<JLaunch> jlaunches;
     std::vector<VSeg> jsegs;          // count = the scene's elements
-    std::vector<int> jblock_seg;
     std::vector<BatchSeg> tsegs;      // element tasks: owners ranges
-    std::vector<int> tblock_seg;
     q2::Basis1D q2_basis{};
     int max_iterations = 0;
     int64_t vertex_updates_per_substep = 0;
@@ -66,13 +66,9 @@ struct Schedule {
 // Device copies of a schedule's tables (grown on demand, reused across uploads).
 struct DeviceSchedule {
     DBuf<BatchSeg> segs;
-    DBuf<int> block_seg;
     DBuf<VSeg> vsegs;
-    DBuf<int> vblock_seg;
     DBuf<VSeg> jsegs;
-    DBuf<int> jblock_seg;
     DBuf<BatchSeg> tsegs;
-    DBuf<int> tblock_seg;
     void upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync);
 };
 

@@ -237,7 +237,6 @@ void check_tile(int element, int32_t lanes, int32_t qpl, int32_t mode) {
     OE_REQUIRE(mode == oe::PASS_GATHER || mode == oe::PASS_CACHE || mode == oe::PASS_TASKS,
                "mode must be 0 (gather), 1 (cached gradients) or 2 (element tasks)");
     OE_REQUIRE(mode != oe::PASS_CACHE || qpl == 1, "the gradient cache takes one quadrature point per lane");
-    OE_REQUIRE(mode != oe::PASS_TASKS || element == oe::Hexa27, "the element-task pass is for 27-node hexahedra");
 }
 }  // namespace
 
@@ -247,7 +246,8 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
         oe::Scene& sc = *s->scene;
         check_tile(sc.mesh.element, lanes, qpl, mode);
         OE_REQUIRE(mode == oe::PASS_GATHER || sc.vbd_fem, "cached gradients and element tasks are VBD options");
-        OE_REQUIRE(mode != oe::PASS_TASKS || sc.vbd_fem->has_q2_basis, "element tasks unavailable for this element");
+        OE_REQUIRE(mode != oe::PASS_TASKS || sc.mesh.element != oe::Hexa27 || sc.vbd_fem->has_q2_basis,
+                   "no tensor factors for this 27-node hexahedron");
         sc.set_graph(false);
         if (sc.vbd_fem) {
             sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, mode};

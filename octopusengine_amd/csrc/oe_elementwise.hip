@@ -13,6 +13,7 @@
 //   Jacobi colour copy        src/GPU/VBD/GPU_VBD_FEM.cu:402-413
 #include "oe_elementwise.hpp"
 #include "oe_gather.hpp"
+#include "oe_kernels.hpp"
 
 namespace oe {
 
@@ -159,8 +160,9 @@ __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* sl
 // One launch covers every vertex of every scene of a batch; each block belongs
 // to one scene (VSeg), whose description is read with scalar loads.
 __device__ __forceinline__ const SceneDev& vseg_scene(const BatchVArgs& b, int& i, int& ok) {
-    const VSeg seg = b.segs[b.block_seg[blockIdx.x]];
-    i = (blockIdx.x - seg.block_begin) * kBlock + threadIdx.x;
+    const int blk = static_cast<int>(blockIdx.x);
+    const VSeg seg = find_seg(b.segs, 0, b.nsegs, blk);
+    i = (blk - seg.block_begin) * kBlock + threadIdx.x;
     ok = i < seg.count;
     return b.scenes[seg.scene];
 }

@@ -25,7 +25,8 @@ void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int co
                         hipStream_t s);
 
 struct SceneDev;
-// All vertices of one scene, covered by blocks [block_begin, ...) of a launch.
+// All vertices of one scene, covered by blocks [block_begin, next segment's
+// block_begin) of a launch (block_begin is launch-local).
 struct VSeg {
     int scene;
     int count;
@@ -33,8 +34,8 @@ struct VSeg {
 };
 struct BatchVArgs {
     const SceneDev* scenes;
-    const VSeg* segs;
-    const int* block_seg;
+    const VSeg* segs;   // one launch covers segs[0 .. nsegs)
+    int nsegs;
     float dt;
     int iteration;
 };

@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256, gather_min_waves<QL>()) void k_vertex_gather(G
 // once per block with scalar loads.
 template <int S, int Q, int L, int QL, bool CACHED>
 __global__ __launch_bounds__(256, gather_min_waves<QL>()) void k_batch_gather(BatchGatherArgs b) {
-    const int blk = b.block_base + xcd_block(blockIdx.x, gridDim.x);
-    const BatchSeg seg = b.segs[b.block_seg[blk]];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;
     GatherArgs a;
@@ -272,13 +272,63 @@ __global__ __launch_bounds__(256) void k_ju_refresh(PSView ps, FemView f, float*
 template <int S, int Q>
 __global__ __launch_bounds__(256) void k_batch_ju_refresh(BatchRefreshArgs b) {
     constexpr int EPB = refresh_epb<Q>();
-    const int blk = b.block_base + xcd_block(blockIdx.x, gridDim.x);
-    const VSeg seg = b.segs[b.block_seg[blk]];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const VSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;      // uniform over the block
     const int e0 = (blk - seg.block_begin) * EPB;
     const int ne = min(EPB, seg.count - e0);
     ju_refresh_block<S, Q>(sd.ps.p, sd.fem, sd.ju, e0, ne);
+}
+
+// Element task (one lane): all Q points of element e, contribution to its
+// local node r, 9-float partial at owners index first + task.
+template <int S, int Q>
+__device__ __forceinline__ void elem_task(const PSView& ps, const FemView& f, const MaterialView& mt,
+                                          const int* __restrict__ own, float* partial, int first, int count,
+                                          int task) {
+    if (task >= count) return;
+    const int pk = own[first + task];
+    const int eid = pk >> 5;
+    const int r = pk & 31;
+    const int* __restrict__ tp = f.topo + static_cast<size_t>(eid) * S;
+    if (ps.mask[tp[r]] == 0) return;
+    float xs[S][3];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const float4 x = ps.p[tp[k]];
+        xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+    }
+    Acc9 acc;
+    acc_zero(acc);
+    // points one after the other: keeps the live state to one point's
+#pragma unroll 1
+    for (int q = 0; q < Q; ++q) {
+        const float* __restrict__ dn = f.dN + q * S * 3;
+        float Ju[9], JXi[9], V;
+        load_element_record(f, eid, q, JXi, V);
+        jx_from_nodes<S>(xs, dn, Ju);
+        nh_contrib(Ju, JXi, V, dn + 3 * r, mt, acc);
+    }
+    float* out = partial + static_cast<size_t>(first + task) * 9;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = acc.v[i];
+}
+
+template <int S, int Q>
+__global__ __launch_bounds__(kElemTasksPerBlock) void k_elem_tasks(ElemTaskArgs a) {
+    const int task = xcd_block(blockIdx.x, gridDim.x) * kElemTasksPerBlock + threadIdx.x;
+    elem_task<S, Q>(a.ps, a.fem, a.mt, a.own, a.partial, a.first, a.count, task);
+}
+
+template <int S, int Q>
+__global__ __launch_bounds__(kElemTasksPerBlock) void k_batch_elem_tasks(BatchElemTaskArgs b) {
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
+    const SceneDev& sd = b.scenes[seg.scene];
+    if (b.iteration >= sd.iterations) return;
+    const int task = (blk - seg.block_begin) * kElemTasksPerBlock + threadIdx.x;
+    elem_task<S, Q>(sd.ps, sd.fem, sd.mt, sd.own.own, sd.partial, seg.first, seg.count, task);
 }
 
 namespace {
@@ -370,6 +420,19 @@ void OE_FN(launch_ju_refresh_)(const PSView& ps, const FemView& f, float* ju, hi
 }
 
 int OE_FN(refresh_elements_per_block_)() { return refresh_epb<OE_GATHER_Q>(); }
+
+void OE_FN(launch_elem_tasks_)(const ElemTaskArgs& a, hipStream_t s) {
+    if (a.count <= 0) return;
+    hipLaunchKernelGGL((k_elem_tasks<OE_GATHER_S, OE_GATHER_Q>),
+                       dim3((a.count + kElemTasksPerBlock - 1) / kElemTasksPerBlock), dim3(kElemTasksPerBlock), 0, s,
+                       a);
+}
+
+void OE_FN(launch_batch_elem_tasks_)(const BatchElemTaskArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL((k_batch_elem_tasks<OE_GATHER_S, OE_GATHER_Q>), dim3(nblocks), dim3(kElemTasksPerBlock), 0, s,
+                       b);
+}
 
 void OE_FN(launch_batch_ju_refresh_)(const BatchRefreshArgs& b, int nblocks, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;

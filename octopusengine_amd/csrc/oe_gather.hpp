@@ -60,7 +60,7 @@ struct SceneDev {
 };
 
 // A run of `count` consecutive slots (first = first slot) of one scene, covered
-// by blocks [block_begin, block_begin + nblocks) of a launch.
+// by the launch-local blocks [block_begin, next segment's block_begin).
 struct BatchSeg {
     int scene;
     int first;
@@ -71,11 +71,33 @@ struct BatchSeg {
 struct BatchGatherArgs {
     const SceneDev* scenes;
     const BatchSeg* segs;
-    const int* block_seg;   // block -> segment id
-    int block_base;         // first entry of block_seg used by this launch
+    int seg_first;          // this launch's segments: segs[seg_first .. seg_first + seg_count)
+    int seg_count;
     int iteration;          // scenes with fewer iterations skip the launch
     float dt;
 };
+
+// Element tasks for the task form of a VBD colour pass: one lane per owners
+// entry (element e, local node r) of the colour, all Q points of e, partial
+// force / Hessian of node r (9 floats) written at the owners index.
+struct ElemTaskArgs {
+    PSView ps;
+    FemView fem;
+    MaterialView mt;
+    const int* own;
+    float* partial;
+    int first;   // first owners index of the colour
+    int count;   // incidences of the colour
+};
+// Batched: BatchSeg.first / count are an owners range of one scene.
+struct BatchElemTaskArgs {
+    const SceneDev* scenes;
+    const BatchSeg* segs;
+    int seg_first;
+    int seg_count;
+    int iteration;
+};
+constexpr int kElemTasksPerBlock = 256;
 
 // Batched refresh of the cached displacement gradients: VSeg.count is the
 // scene's element count; each block takes refresh_elements_per_block(S, Q)
@@ -83,8 +105,8 @@ struct BatchGatherArgs {
 struct BatchRefreshArgs {
     const SceneDev* scenes;
     const VSeg* segs;
-    const int* block_seg;
-    int block_base;
+    int seg_first;
+    int seg_count;
     int iteration;
 };
 
@@ -94,7 +116,9 @@ struct BatchRefreshArgs {
                                        hipStream_t s);                                                        \
     void launch_ju_refresh_##S##_##Q(const PSView& ps, const FemView& f, float* ju, hipStream_t s);          \
     void launch_batch_ju_refresh_##S##_##Q(const BatchRefreshArgs& b, int nblocks, hipStream_t s);          \
-    int refresh_elements_per_block_##S##_##Q();
+    int refresh_elements_per_block_##S##_##Q();                                                               \
+    void launch_elem_tasks_##S##_##Q(const ElemTaskArgs& a, hipStream_t s);                                   \
+    void launch_batch_elem_tasks_##S##_##Q(const BatchElemTaskArgs& b, int nblocks, hipStream_t s);
 OE_GATHER_DECL(4, 1)
 OE_GATHER_DECL(8, 8)
 OE_GATHER_DECL(10, 4)
@@ -118,6 +142,26 @@ inline bool launch_ju_refresh(int S, int Q, const PSView& ps, const FemView& f, 
     else if (S == 10 && Q == 4) launch_ju_refresh_10_4(ps, f, ju, s);
     else if (S == 20 && Q == 11) launch_ju_refresh_20_11(ps, f, ju, s);
     else if (S == 27 && Q == 27) launch_ju_refresh_27_27(ps, f, ju, s);
+    else return false;
+    return true;
+}
+
+inline bool launch_elem_tasks(int S, int Q, const ElemTaskArgs& a, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_elem_tasks_4_1(a, s);
+    else if (S == 8 && Q == 8) launch_elem_tasks_8_8(a, s);
+    else if (S == 10 && Q == 4) launch_elem_tasks_10_4(a, s);
+    else if (S == 20 && Q == 11) launch_elem_tasks_20_11(a, s);
+    else if (S == 27 && Q == 27) launch_elem_tasks_27_27(a, s);
+    else return false;
+    return true;
+}
+
+inline bool launch_batch_elem_tasks(int S, int Q, const BatchElemTaskArgs& b, int nblocks, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_batch_elem_tasks_4_1(b, nblocks, s);
+    else if (S == 8 && Q == 8) launch_batch_elem_tasks_8_8(b, nblocks, s);
+    else if (S == 10 && Q == 4) launch_batch_elem_tasks_10_4(b, nblocks, s);
+    else if (S == 20 && Q == 11) launch_batch_elem_tasks_20_11(b, nblocks, s);
+    else if (S == 27 && Q == 27) launch_batch_elem_tasks_27_27(b, nblocks, s);
     else return false;
     return true;
 }

@@ -368,9 +368,15 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
         }
         if (by_tasks) {
             const int first_own = a.count > 0 ? owners.slot_off[a.first_slot] : 0;
-            launch_q2_tasks(Q2TaskArgs{a.ps, a.fem, a.mt, q2_basis, d_own.get(), partial.get(), first_own,
-                                       static_cast<int>(owners.color_incidences[c])},
-                            s);
+            const int ntask = static_cast<int>(owners.color_incidences[c]);
+            if (element == Hexa27) {
+                launch_q2_tasks(Q2TaskArgs{a.ps, a.fem, a.mt, q2_basis, d_own.get(), partial.get(), first_own, ntask},
+                                s);
+            } else if (!launch_elem_tasks(S, Q, ElemTaskArgs{a.ps, a.fem, a.mt, d_own.get(), partial.get(), first_own,
+                                                             ntask},
+                                          s)) {
+                throw std::invalid_argument("unsupported element");
+            }
             launch_vertex_solve(VertexSolveArgs{a.ps, a.own, y, partial.get(), damping, dt, a.first_slot, a.count}, s);
         } else if (!launch_gather(S, Q, tile.L, tile.QL, a, s)) {
             throw std::invalid_argument("unsupported gather tile shape");

@@ -129,8 +129,9 @@ enum VBD_Version : int { Base = 0, Threaded_Quadrature = 1, Reduction_Symmetry =
 //   gather  one kernel, L lanes per vertex, QL quadrature points per lane;
 //   cache   the same with the displacement gradients of every (element,
 //           quadrature point) kept in device memory between colours (QL = 1);
-//   tasks   27-node hexahedra only: element tasks with sum-factorised
-//           gradients, then a vertex-solve kernel (L, QL unused).
+//   tasks   element tasks (one per incidence of the colour; sum-factorised
+//           gradients for 27-node hexahedra), then a vertex-solve kernel
+//           (L, QL unused).
 enum PassMode : int { PASS_GATHER = 0, PASS_CACHE = 1, PASS_TASKS = 2 };
 struct GatherTile {
     int L = 16;
@@ -170,7 +171,9 @@ struct GPU_VBD_FEM final : GPU_FEM {
     // element-task partials, 9 floats per owners entry (allocated on first use)
     DBuf<float> partial;
     float* ensure_partial();
-    bool tasks() const { return tile.mode == PASS_TASKS && has_q2_basis && version != Better_Coloration; }
+    bool tasks() const {
+        return tile.mode == PASS_TASKS && version != Better_Coloration && (element != Hexa27 || has_q2_basis);
+    }
     bool has_q2_basis = false;
     q2::Basis1D q2_basis{};
     int64_t conflicts = 0;

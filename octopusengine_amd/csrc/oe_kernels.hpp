@@ -25,6 +25,22 @@ __device__ __forceinline__ int xcd_block(int bid, int n) {
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+// Segment of a batched launch that holds launch-local block b: the launch's
+// segments segs[first .. first + n) have ascending block_begin (the first is
+// 0); binary search, uniform over the block.
+template <typename Seg>
+__device__ __forceinline__ const Seg& find_seg(const Seg* __restrict__ segs, int first, int n, int b) {
+    int lo = first, hi = first + n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block_begin <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return segs[lo];
+}
+
 // Per-vertex accumulator: force (3) + lower triangle of the symmetric 3x3
 // Hessian block (H00, H10, H20, H11, H21, H22).
 struct Acc9 {

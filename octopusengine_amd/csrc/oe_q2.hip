@@ -147,8 +147,8 @@ __global__ __launch_bounds__(64) void k_q2_tasks(Q2TaskArgs a) {
 
 __global__ __launch_bounds__(64) void k_batch_q2_tasks(BatchQ2TaskArgs b) {
     __shared__ TaskLDS sh[kTasksPerBlock];
-    const int blk = b.block_base + xcd_block(blockIdx.x, gridDim.x);
-    const BatchSeg seg = b.segs[b.block_seg[blk]];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;     // uniform over the block
     const int half = threadIdx.x >> 5;
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(256) void k_vertex_solve(VertexSolveArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_batch_vertex_solve(BatchGatherArgs b) {
-    const int blk = b.block_base + blockIdx.x;
-    const BatchSeg seg = b.segs[b.block_seg[blk]];
+    const int blk = static_cast<int>(blockIdx.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;
     const int i = (blk - seg.block_begin) * 256 + threadIdx.x;

@@ -37,8 +37,8 @@ struct VertexSolveArgs {
 struct BatchQ2TaskArgs {
     const SceneDev* scenes;
     const BatchSeg* segs;
-    const int* block_seg;
-    int block_base;
+    int seg_first;
+    int seg_count;
     int iteration;
     q2::Basis1D basis;
 };

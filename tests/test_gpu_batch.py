@@ -52,8 +52,8 @@ def test_batch_equals_alone_and_oracle(engine):
 def test_batch_tiles_with_gradient_cache(engine):
     """Per-element-type tiles, with and without the gradient cache, give each
     scene the step it gets alone with the same tile."""
-    tiles = {oe.TETRA10: (16, 1, oe.PASS_CACHE), oe.HEXA27: (32, 1, oe.PASS_TASKS), oe.TETRA: (8, 1, oe.PASS_CACHE),
-             oe.TETRA20: (64, 1, oe.PASS_CACHE), oe.HEXA: (16, 1, oe.PASS_CACHE)}
+    tiles = {oe.TETRA10: (16, 1, oe.PASS_TASKS), oe.HEXA27: (32, 1, oe.PASS_TASKS), oe.TETRA: (8, 1, oe.PASS_CACHE),
+             oe.TETRA20: (64, 1, oe.PASS_TASKS), oe.HEXA: (16, 1, oe.PASS_CACHE)}
     batched = [_make(engine, s)[0] for s in MIX]
     alone = [_make(engine, s, tile=tiles[s["el"]])[0] for s in MIX]
     b = oe.Batch(engine, batched)

@@ -137,6 +137,25 @@ def test_vbd_q2_element_tasks(engine, curve):
     assert pair <= TOL1 / 10
 
 
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+def test_vbd_element_tasks(engine, case):
+    """Task form of the colour pass for every element type: same step as the
+    gather form and the oracle."""
+    import octopusengine_amd as oe
+
+    el, cells = case
+    g, o, _, _ = make_pair(engine, el, cells, tile=(16, 1, oe.PASS_TASKS))
+    r, _, _, _ = make_pair(engine, el, cells, tile=(16, 1, oe.PASS_GATHER))
+    for _ in range(2):
+        g.step(0.005, 1)
+        r.step(0.005, 1)
+        o.step(0.005, 1)
+    e_o = max_err(g.positions(), o.positions())
+    e_r = max_err(g.positions(), r.positions())
+    print(f"tasks el={el} err oracle {e_o:.3e} gather {e_r:.3e}", flush=True)
+    assert e_o <= TOL1 and e_r <= TOL1 / 10
+
+
 def test_vbd_q2_element_tasks_many_steps(engine):
     import octopusengine_amd as oe
 
@@ -156,7 +175,7 @@ def test_gradient_cache_tile_checks(engine):
 
     g, _, _, _ = make_pair(engine, 7, (2, 1, 1))
     with pytest.raises(oe.OEError):
-        g.set_tile(16, 1, oe.PASS_TASKS)  # element tasks are for 27-node hexahedra
+        g.set_tile(16, 1, 3)             # no such mode
     with pytest.raises(oe.OEError):
         g.set_tile(16, 2, True)          # the cache takes one quadrature point per lane
     x, _, _, _ = make_pair(engine, 7, (2, 1, 1), solver=oe.SOLVER_EXPLICIT, iterations=1)
