@@ -26,9 +26,6 @@ GPU_Box_Limit* find_box(const Scene& s) {
 std::map<int, GatherTile> default_tiles() {
     std::map<int, GatherTile> t;
     for (Element e : {Tetra, Hexa, Tetra10, Tetra20, Hexa27}) t[e] = default_tile(e);
-    // many small P2 scenes per launch favour narrower vertex groups than one
-    // large scene does (measured on the 256-scene batch)
-    t[Tetra10] = GatherTile{8, 2, PASS_GATHER};
     return t;
 }
 
@@ -65,7 +62,13 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
     if (t.mode == PASS_TASKS && s.mesh.element == Hexa27 && !s.vbd_fem->has_q2_basis)
         throw std::invalid_argument("no tensor factors for this 27-node hexahedron");
     d.ju = t.mode == PASS_CACHE ? s.vbd_fem->ensure_ju() : nullptr;
-    d.partial = t.mode == PASS_TASKS ? s.vbd_fem->ensure_partial() : nullptr;
+    d.partial = nullptr;
+    d.fblocks = nullptr;
+    if (t.mode == PASS_TASKS) {
+        s.vbd_fem->prepare_tasks();
+        d.partial = s.vbd_fem->partial.get();
+        d.fblocks = s.vbd_fem->d_fblk.get();
+    }
     d.ps = s.ps->get_parameters();
     d.fem = s.fem->fem_view();
     d.own = OwnersView{s.vbd_fem->d_slot_vid.get(), s.vbd_fem->d_slot_nbo.get(), s.vbd_fem->d_own.get()};
@@ -141,14 +144,23 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     any = true;
                     if (gl.mode == PASS_TASKS) q2_basis = s->vbd_fem->q2_basis;
                 }
-                // gather: L lanes per slot; tasks: the vertex solve takes one thread per slot
-                const int lanes = gl.mode == PASS_TASKS ? 1 : gl.L;
-                const int nb = static_cast<int>((static_cast<int64_t>(count) * lanes + kBlock - 1) / kBlock);
-                segs.push_back(BatchSeg{table_index[i], first, count, gl.nblocks});
-                gl.nblocks += nb;
-                if (gl.mode == PASS_TASKS) {
+                if (gl.mode == PASS_TASKS && el != Hexa27) {
+                    // fused tasks: one block per vertex block of the colour
+                    const std::vector<int32_t>& fb = s->vbd_fem->fblk_color_begin;
+                    if (fb.empty()) throw std::logic_error("vertex blocks not prepared");
+                    const int nfb = fb[c + 1] - fb[c];
+                    segs.push_back(BatchSeg{table_index[i], fb[c], nfb, gl.nblocks});
+                    gl.nblocks += nfb;
+                } else {
+                    // gather: L lanes per slot; Q2 tasks: the vertex solve takes one thread per slot
+                    const int lanes = gl.mode == PASS_TASKS ? 1 : gl.L;
+                    const int nb = static_cast<int>((static_cast<int64_t>(count) * lanes + kBlock - 1) / kBlock);
+                    segs.push_back(BatchSeg{table_index[i], first, count, gl.nblocks});
+                    gl.nblocks += nb;
+                }
+                if (gl.mode == PASS_TASKS && el == Hexa27) {
                     const int tasks = static_cast<int>(s->vbd_fem->owners.color_incidences[c]);
-                    const int tpb = el == Hexa27 ? q2_tasks_per_block() : kElemTasksPerBlock;
+                    const int tpb = q2_tasks_per_block();
                     if (tasks > 0) {
                         tsegs.push_back(BatchSeg{table_index[i], s->vbd_fem->owners.slot_off[first], tasks,
                                                  gl.tnblocks});
@@ -249,11 +261,11 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
     BatchGatherArgs g{d_scenes, dev.segs.get(), 0, 0, 0, h};
     BatchRefreshArgs r{d_scenes, dev.jsegs.get(), 0, 0, 0};
     BatchQ2TaskArgs tk{d_scenes, dev.tsegs.get(), 0, 0, 0, sch.q2_basis};
-    BatchElemTaskArgs et{d_scenes, dev.tsegs.get(), 0, 0, 0};
+    BatchFusedArgs ft{d_scenes, dev.segs.get(), 0, 0, 0, h};
     for (int sub = 0; sub < substeps; ++sub) {
         launch_batch_predict(v, sch.vblocks, stream);
         for (int it = 0; it < sch.max_iterations; ++it) {
-            g.iteration = r.iteration = tk.iteration = et.iteration = it;
+            g.iteration = r.iteration = tk.iteration = ft.iteration = it;
             for (const Schedule::JLaunch& jl : sch.jlaunches) {
                 r.seg_first = jl.seg_first;
                 r.seg_count = jl.seg_count;
@@ -264,18 +276,16 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                 g.seg_first = gl.seg_first;
                 g.seg_count = gl.seg_count;
                 if (timer) timer->before(stream);
-                if (gl.mode == PASS_TASKS) {
-                    if (gl.S == 27 && gl.Q == 27) {
-                        tk.seg_first = gl.tseg_first;
-                        tk.seg_count = gl.tseg_count;
-                        if (gl.tseg_count > 0) launch_batch_q2_tasks(tk, gl.tnblocks, stream);
-                    } else if (gl.tseg_count > 0) {
-                        et.seg_first = gl.tseg_first;
-                        et.seg_count = gl.tseg_count;
-                        if (!launch_batch_elem_tasks(gl.S, gl.Q, et, gl.tnblocks, stream))
-                            throw std::invalid_argument("unsupported element in batch");
-                    }
+                if (gl.mode == PASS_TASKS && gl.S == 27 && gl.Q == 27) {
+                    tk.seg_first = gl.tseg_first;
+                    tk.seg_count = gl.tseg_count;
+                    if (gl.tseg_count > 0) launch_batch_q2_tasks(tk, gl.tnblocks, stream);
                     launch_batch_vertex_solve(g, gl.nblocks, stream);
+                } else if (gl.mode == PASS_TASKS) {
+                    ft.seg_first = gl.seg_first;
+                    ft.seg_count = gl.seg_count;
+                    if (!launch_batch_fused_tasks(gl.S, gl.Q, ft, gl.nblocks, stream))
+                        throw std::invalid_argument("unsupported element in batch");
                 } else if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.mode == PASS_CACHE, g, gl.nblocks,
                                                 stream)) {
                     throw std::invalid_argument("unsupported gather tile in batch");

@@ -250,9 +250,15 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
                    "no tensor factors for this 27-node hexahedron");
         sc.set_graph(false);
         if (sc.vbd_fem) {
+            const oe::GatherTile before = sc.vbd_fem->tile;
             sc.vbd_fem->tile = oe::GatherTile{lanes, qpl, mode};
-            if (sc.vbd_fem->cached()) sc.vbd_fem->ensure_ju();
-            if (sc.vbd_fem->tasks()) sc.vbd_fem->ensure_partial();
+            try {
+                if (sc.vbd_fem->cached()) sc.vbd_fem->ensure_ju();
+                if (sc.vbd_fem->tasks()) sc.vbd_fem->prepare_tasks();
+            } catch (...) {
+                sc.vbd_fem->tile = before;
+                throw;
+            }
         }
         if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, oe::PASS_GATHER};
     });

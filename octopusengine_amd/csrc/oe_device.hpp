@@ -162,14 +162,45 @@ struct FemView {
     const float* wq;     // Q
 };
 
+// Loads through pointers known to address device (global) memory.  Every
+// buffer a view points at is a device allocation; a pointer read from a table
+// in memory is otherwise a generic one, whose loads the hardware counts
+// together with LDS operations, so that a wait for one waits for all.
+typedef float oe_float4_vec __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ __forceinline__ T gload(const T* p, size_t i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+__device__ __forceinline__ float4 gload(const float4* p, size_t i) {
+    const oe_float4_vec v = ((const __attribute__((address_space(1))) oe_float4_vec*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+typedef int oe_int2_vec __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int2 gload(const int2* p, size_t i) {
+    const oe_int2_vec v = ((const __attribute__((address_space(1))) oe_int2_vec*)p)[i];
+    return make_int2(v.x, v.y);
+}
+typedef int oe_int4_vec __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 gload(const int4* p, size_t i) {
+    const oe_int4_vec v = ((const __attribute__((address_space(1))) oe_int4_vec*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+template <class T>
+__device__ __forceinline__ void gstore(T* p, size_t i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+__device__ __forceinline__ void gstore(float4* p, size_t i, float4 v) {
+    const oe_float4_vec w = {v.x, v.y, v.z, v.w};
+    ((__attribute__((address_space(1))) oe_float4_vec*)p)[i] = w;
+}
+
 // Element record of quadrature point q of element e in either layout
 // (branch-free: the layout only changes the record index).
 __device__ __forceinline__ void load_element_record(const FemView& f, int e, int q, float JXi[9], float& V) {
     const size_t idx = f.affine ? static_cast<size_t>(e) : static_cast<size_t>(e) * f.Q + q;
-    const float* __restrict__ r = f.jxv + idx * 10;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) JXi[i] = r[i];
-    V = r[9] * f.wq[q];
+    for (int i = 0; i < 9; ++i) JXi[i] = gload(f.jxv, idx * 10 + i);
+    V = gload(f.jxv, idx * 10 + 9) * gload(f.wq, q);
 }
 
 // Colour-major owners CSR (reference GPU_Owners_Parameters,

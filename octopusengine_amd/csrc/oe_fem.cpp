@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <string>
 
 namespace oe {
 
@@ -238,7 +239,7 @@ void lame_parameters(float young, float poisson, float* lambda, float* mu) {
     *mu = young / (2.f * (1.f + poisson));
 }
 
-OwnersCSR build_owners(const HostMesh& m, const Coloration& col) {
+OwnersCSR build_owners(const HostMesh& m, const Coloration& col, bool by_valence) {
     const int N = m.nb_vertices();
     const int S = elem_nb_vertices(m.element);
     const int E = m.nb_elements();
@@ -270,6 +271,14 @@ OwnersCSR build_owners(const HostMesh& m, const Coloration& col) {
     std::vector<int32_t> cur(o.color_begin.begin(), o.color_begin.end() - 1);
     o.slot_vid.resize(N);
     for (int v = 0; v < N; ++v) o.slot_vid[cur[col.colors[v]]++] = v;
+    if (by_valence) {
+        // Within a colour the vertices are independent, so their order does not
+        // change any result; most incidences first groups vertices of equal
+        // work into the same wavefront.
+        for (int c = 0; c < nc; ++c)
+            std::stable_sort(o.slot_vid.begin() + o.color_begin[c], o.slot_vid.begin() + o.color_begin[c + 1],
+                             [&](int a, int b) { return cnt[a + 1] - cnt[a] > cnt[b + 1] - cnt[b]; });
+    }
     o.slot_nb.resize(N);
     o.slot_off.resize(N);
     o.own.reserve(m.topo.size());
@@ -283,6 +292,34 @@ OwnersCSR build_owners(const HostMesh& m, const Coloration& col) {
         o.color_incidences[c] += o.slot_nb[s];
     }
     return o;
+}
+
+void build_vertex_blocks(const OwnersCSR& o, int max_inc, int max_slots, std::vector<int32_t>& blocks,
+                         std::vector<int32_t>& color_begin) {
+    const int nc = static_cast<int>(o.color_begin.size()) - 1;
+    blocks.clear();
+    color_begin.assign(nc + 1, 0);
+    for (int c = 0; c < nc; ++c) {
+        color_begin[c] = static_cast<int32_t>(blocks.size() / 4);
+        int first = -1, n_slots = 0, n_inc = 0;
+        auto flush = [&] {
+            if (n_slots > 0) blocks.insert(blocks.end(), {first, n_slots, o.slot_off[first], n_inc});
+            first = -1;
+            n_slots = n_inc = 0;
+        };
+        for (int s = o.color_begin[c]; s < o.color_begin[c + 1]; ++s) {
+            const int nb = o.slot_nb[s];
+            if (nb > max_inc)
+                throw std::invalid_argument("a vertex has " + std::to_string(nb) + " incident elements, more than the " +
+                                            std::to_string(max_inc) + " of a fused task block");
+            if (n_slots == max_slots || n_inc + nb > max_inc) flush();
+            if (n_slots == 0) first = s;
+            n_slots++;
+            n_inc += nb;
+        }
+        flush();
+    }
+    color_begin[nc] = static_cast<int32_t>(blocks.size() / 4);
 }
 
 ElementSchedule build_element_schedule(const Coloration& col, int nb_elements) {

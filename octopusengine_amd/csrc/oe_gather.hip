@@ -26,6 +26,7 @@
 // This file is compiled once per element type (OE_GATHER_S / OE_GATHER_Q set
 // on the command line) so that the template instances build in parallel.
 #include "oe_gather.hpp"
+#include "oe_host.hpp"
 #include "oe_kernels.hpp"
 
 namespace oe {
@@ -42,9 +43,16 @@ constexpr int kept_items() {
 __device__ __forceinline__ void work_item(const int* __restrict__ own, int t, int QC, int& eid, int& qc, int& r) {
     const int j = t / QC;
     qc = t - j * QC;
-    const int pk = own[j];
+    const int pk = gload(own, j);
     eid = pk >> 5;
     r = pk & 31;
+}
+
+// dN_r at one point, from the shape-derivative table in device memory
+__device__ __forceinline__ void load_dnr(const float* dn, int at, float dnr[3]) {
+    dnr[0] = gload(dn, at);
+    dnr[1] = gload(dn, at + 1);
+    dnr[2] = gload(dn, at + 2);
 }
 
 // column-major: Ju[c*3 + k] += du[k] dN_r[c]
@@ -67,9 +75,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
     constexpr int KEEP = CACHED ? kept_items<Q, L>() : 1;
     if (grp >= a.count) return;                 // whole group leaves together
     const int slot = a.first_slot + grp;
-    const int vid = a.own.slot_vid[slot];
-    if (a.mode != GATHER_EXPLICIT_FORCE && a.ps.mask[vid] == 0) return;
-    const int2 nbo = a.own.slot_nbo[slot];
+    const int vid = gload(a.own.slot_vid, slot);
+    if (a.mode != GATHER_EXPLICIT_FORCE && gload(a.ps.mask, vid) == 0) return;
+    const int2 nbo = gload(a.own.slot_nbo, slot);
     const int nwork = nbo.x * QC;
     const int* __restrict__ own = a.own.own + nbo.y;
     Acc9 acc;
@@ -83,33 +91,37 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
             if (t < nwork) {
                 int eid, q, r;
                 work_item(own, t, Q, eid, q, r);
-                const float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
+                const size_t g = (static_cast<size_t>(eid) * Q + q) * 9;
                 float JXi[9], V;
 #pragma unroll
-                for (int i = 0; i < 9; ++i) keep[it][i] = g[i];
+                for (int i = 0; i < 9; ++i) keep[it][i] = gload(a.ju, g + i);
                 load_element_record(a.fem, eid, q, JXi, V);
-                nh_contrib(keep[it], JXi, V, a.fem.dN + (q * S + r) * 3, a.mt, acc);
+                float dnr[3];
+                load_dnr(a.fem.dN, (q * S + r) * 3, dnr);
+                nh_contrib(keep[it], JXi, V, dnr, a.mt, acc);
             }
         }
         for (int t = lane + KEEP * L; t < nwork; t += L) {
             int eid, q, r;
             work_item(own, t, Q, eid, q, r);
-            const float* g = a.ju + (static_cast<size_t>(eid) * Q + q) * 9;
+            const size_t g = (static_cast<size_t>(eid) * Q + q) * 9;
             float Ju[9], JXi[9], V;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Ju[i] = g[i];
+            for (int i = 0; i < 9; ++i) Ju[i] = gload(a.ju, g + i);
             load_element_record(a.fem, eid, q, JXi, V);
-            nh_contrib(Ju, JXi, V, a.fem.dN + (q * S + r) * 3, a.mt, acc);
+            float dnr[3];
+            load_dnr(a.fem.dN, (q * S + r) * 3, dnr);
+            nh_contrib(Ju, JXi, V, dnr, a.mt, acc);
         }
     } else {
         for (int t = lane; t < nwork; t += L) {
             int eid, qc, r;
             work_item(own, t, QC, eid, qc, r);
-            const int* __restrict__ tp = a.fem.topo + static_cast<size_t>(eid) * S;
+            const size_t tp = static_cast<size_t>(eid) * S;
             float xs[S][3];
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                const float4 x = P[tp[k]];
+                const float4 x = gload(static_cast<const float4*>(P), gload(a.fem.topo, tp + k));
                 xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
             }
 #pragma unroll
@@ -119,7 +131,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
                 float Ju[9], JXi[9], V;
                 load_element_record(a.fem, eid, q, JXi, V);
                 jx_from_nodes<S>(xs, dn, Ju);
-                nh_contrib(Ju, JXi, V, dn + 3 * r, a.mt, acc);
+                float dnr[3];
+                load_dnr(dn, 3 * r, dnr);
+                nh_contrib(Ju, JXi, V, dnr, a.mt, acc);
             }
         }
     }
@@ -144,15 +158,16 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
     }
     float du[3] = {0.f, 0.f, 0.f};
     if (lane == 0) {
-        const float4 pv = P[vid];
+        const float4 pv = gload(static_cast<const float4*>(P), vid);
         double dx[3];
-        vbd_local_solve(acc, pv, a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt, dx);
+        vbd_local_solve(acc, pv, gload(static_cast<const float4*>(a.ps.last_p), vid), gload(a.y, vid),
+                        gload(a.ps.m, vid), a.damping, a.dt, dx);
         const float4 out = make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
                                        pv.z + static_cast<float>(dx[2]), pv.w);
         if (a.mode == GATHER_VBD_JACOBI)
-            a.p_new[vid] = out;
+            gstore(a.p_new, vid, out);
         else
-            a.ps.p[vid] = out;
+            gstore(a.ps.p, vid, out);
         du[0] = out.x - pv.x;
         du[1] = out.y - pv.y;
         du[2] = out.z - pv.z;
@@ -281,55 +296,177 @@ __global__ __launch_bounds__(256) void k_batch_ju_refresh(BatchRefreshArgs b) {
     ju_refresh_block<S, Q>(sd.ps.p, sd.fem, sd.ju, e0, ne);
 }
 
-// Element task (one lane): all Q points of element e, contribution to its
-// local node r, 9-float partial at owners index first + task.
+#if !defined(OE_GATHER_S) || OE_GATHER_S != 27   // 27-node hexahedra have their own task kernels
+// Fused element tasks (see FusedTaskArgs).  Phase 1: thread t of the block
+// takes the incidences t, t + kFusedThreads, ... of the vertex block; the
+// next incidence's owners entry and node indices are loaded while the current
+// one runs.  Fixed vertices are not skipped there: phase 2 skips them.
+//
+// The shape-derivative table is read through an accessor.  Tables of up to
+// 3 KB travel by value in the kernel arguments, so the Jx contraction (the
+// same point q on every lane) reads them with scalar loads; larger ones stay
+// in device memory.  dN_r(q) differs between lanes and always comes from the
+// table in device memory.
 template <int S, int Q>
-__device__ __forceinline__ void elem_task(const PSView& ps, const FemView& f, const MaterialView& mt,
-                                          const int* __restrict__ own, float* partial, int first, int count,
-                                          int task) {
-    if (task >= count) return;
-    const int pk = own[first + task];
-    const int eid = pk >> 5;
-    const int r = pk & 31;
-    const int* __restrict__ tp = f.topo + static_cast<size_t>(eid) * S;
-    if (ps.mask[tp[r]] == 0) return;
-    float xs[S][3];
+struct ShapeGrad {
+    float dN[Q * S * 3];
+};
+template <int S, int Q>
+constexpr bool grad_by_value() {
+    return Q * S * 3 * sizeof(float) <= 3072;
+}
+template <int S, int Q>
+struct GradByValue {
+    const ShapeGrad<S, Q>& g;
+    __device__ __forceinline__ float at(int i) const { return g.dN[i]; }
+};
+struct GradInMemory {
+    const float* dN;
+    __device__ __forceinline__ float at(int i) const { return gload(dN, i); }
+};
+// the kernel-argument table: the element's own, or a 1-point stand-in for
+// types whose table stays in device memory
+template <int S, int Q>
+using ShapeGradArg = ShapeGrad<S, (grad_by_value<S, Q>() ? Q : 1)>;
+
+template <int S, class Grad>
+__device__ __forceinline__ void jx_from_grad(const float (&xs)[S][3], const Grad& grad, int base, float Jx[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Jx[i] = 0.f;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const float4 x = ps.p[tp[k]];
-        xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+        const float d0 = grad.at(base + 3 * k), d1 = grad.at(base + 3 * k + 1), d2 = grad.at(base + 3 * k + 2);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Jx[0 + c] = fmaf(xs[k][c], d0, Jx[0 + c]);
+            Jx[3 + c] = fmaf(xs[k][c], d1, Jx[3 + c]);
+            Jx[6 + c] = fmaf(xs[k][c], d2, Jx[6 + c]);
+        }
     }
+}
+
+// Partial (force, Hessian) of local node r of element eid over its Q points.
+template <int S, int Q, class Grad>
+__device__ __forceinline__ void elem_partial(const FemView& f, const MaterialView& mt, const Grad& grad, int eid,
+                                             int r, const float (&xs)[S][3], Acc9& acc) {
+    acc_zero(acc);
+    if (f.affine) {
+        // one record per element: JX_inv and |det B|, V_q = |det B| w_q
+        float JXi[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) JXi[i] = gload(f.jxv, static_cast<size_t>(eid) * 10 + i);
+        const float detB = gload(f.jxv, static_cast<size_t>(eid) * 10 + 9);
+        // one point at a time: its 3S table entries fit the scalar registers
+#pragma unroll 1
+        for (int q = 0; q < Q; ++q) {
+            float Ju[9], dnr[3];
+            jx_from_grad<S>(xs, grad, q * S * 3, Ju);
+            load_dnr(f.dN, (q * S + r) * 3, dnr);
+            nh_contrib(Ju, JXi, detB * gload(f.wq, q), dnr, mt, acc);
+        }
+    } else {
+#pragma unroll 1
+        for (int q = 0; q < Q; ++q) {
+            float Ju[9], JXi[9], V, dnr[3];
+            load_element_record(f, eid, q, JXi, V);
+            jx_from_grad<S>(xs, grad, q * S * 3, Ju);
+            load_dnr(f.dN, (q * S + r) * 3, dnr);
+            nh_contrib(Ju, JXi, V, dnr, mt, acc);
+        }
+    }
+}
+
+template <int S, int Q, class Grad>
+__device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, const OwnersView& own,
+                                            const MaterialView& mt, const Grad& grad, const float4* y, float damping,
+                                            float dt, const int4 blk, float (*part)[9]) {
+    const int first_slot = blk.x, n_slots = blk.y, first_own = blk.z, n_inc = blk.w;
+    // phase 1: partials of the block's incidences into LDS
+    const int t0 = threadIdx.x;
+    if (t0 < n_inc) {
+        const int last = first_own + n_inc - 1;
+        int pk = gload(own.own, first_own + t0);
+        int tp[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
+        int pk_next = gload(own.own, min(first_own + t0 + kFusedThreads, last));
+#pragma unroll 1
+        for (int t = t0; t < n_inc; t += kFusedThreads) {
+            float xs[S][3];
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const float4 x = gload(static_cast<const float4*>(ps.p), tp[k]);
+                xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+            }
+            const int eid = pk >> 5, r = pk & 31;
+            pk = pk_next;
+#pragma unroll
+            for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
+            pk_next = gload(own.own, min(first_own + t + 2 * kFusedThreads, last));
+            Acc9 acc;
+            elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) part[t][i] = acc.v[i];
+        }
+    }
+    __syncthreads();
+    // phase 2: one thread per slot, partials in the owners order, local solve
+    if (t0 >= n_slots) return;
+    const int slot = first_slot + t0;
+    const int vid = gload(own.slot_vid, slot);
+    if (gload(ps.mask, vid) == 0) return;
+    const int2 nbo = gload(own.slot_nbo, slot);
     Acc9 acc;
     acc_zero(acc);
-    // points one after the other: keeps the live state to one point's
-#pragma unroll 1
-    for (int q = 0; q < Q; ++q) {
-        const float* __restrict__ dn = f.dN + q * S * 3;
-        float Ju[9], JXi[9], V;
-        load_element_record(f, eid, q, JXi, V);
-        jx_from_nodes<S>(xs, dn, Ju);
-        nh_contrib(Ju, JXi, V, dn + 3 * r, mt, acc);
-    }
-    float* out = partial + static_cast<size_t>(first + task) * 9;
+    for (int j = nbo.y - first_own, e = j + nbo.x; j < e; ++j) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) out[i] = acc.v[i];
+        for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
+    }
+    const float4 pv = gload(static_cast<const float4*>(ps.p), vid);
+    double dx[3];
+    vbd_local_solve(acc, pv, gload(static_cast<const float4*>(ps.last_p), vid), gload(y, vid), gload(ps.m, vid),
+                    damping, dt, dx);
+    gstore(ps.p, vid,
+           make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
+                       pv.z + static_cast<float>(dx[2]), pv.w));
 }
 
 template <int S, int Q>
-__global__ __launch_bounds__(kElemTasksPerBlock) void k_elem_tasks(ElemTaskArgs a) {
-    const int task = xcd_block(blockIdx.x, gridDim.x) * kElemTasksPerBlock + threadIdx.x;
-    elem_task<S, Q>(a.ps, a.fem, a.mt, a.own, a.partial, a.first, a.count, task);
+__device__ __forceinline__ void fused_dispatch(const PSView& ps, const FemView& f, const OwnersView& own,
+                                               const MaterialView& mt, const ShapeGradArg<S, Q>& g, const float4* y,
+                                               float damping, float dt, const int4 blk, float (*part)[9]) {
+    if constexpr (grad_by_value<S, Q>())
+        fused_block<S, Q>(ps, f, own, mt, GradByValue<S, Q>{g}, y, damping, dt, blk, part);
+    else
+        fused_block<S, Q>(ps, f, own, mt, GradInMemory{f.dN}, y, damping, dt, blk, part);
 }
 
 template <int S, int Q>
-__global__ __launch_bounds__(kElemTasksPerBlock) void k_batch_elem_tasks(BatchElemTaskArgs b) {
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
+__global__ __launch_bounds__(kFusedThreads) void k_fused_tasks(FusedTaskArgs a, ShapeGradArg<S, Q> g) {
+    __shared__ float part[kFusedIncPerBlock][9];
+    const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
+    fused_dispatch<S, Q>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part);
+}
+
+template <int S, int Q>
+__global__ __launch_bounds__(kFusedThreads) void k_batch_fused_tasks(BatchFusedArgs b, ShapeGradArg<S, Q> g) {
+    __shared__ float part[kFusedIncPerBlock][9];
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, bid);
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;
-    const int task = (blk - seg.block_begin) * kElemTasksPerBlock + threadIdx.x;
-    elem_task<S, Q>(sd.ps, sd.fem, sd.mt, sd.own.own, sd.partial, seg.first, seg.count, task);
+    // the scene's views, read before the first store
+    const PSView ps = sd.ps;
+    const FemView fem = sd.fem;
+    const OwnersView own = sd.own;
+    const MaterialView mt = sd.mt;
+    const float4* y = sd.y;
+    const float damping = sd.damping;
+    const int4 blk = gload(sd.fblocks, seg.first + (bid - seg.block_begin));
+    fused_dispatch<S, Q>(ps, fem, own, mt, g, y, damping, b.dt, blk, part);
 }
+
+#endif
 
 namespace {
 
@@ -421,18 +558,39 @@ void OE_FN(launch_ju_refresh_)(const PSView& ps, const FemView& f, float* ju, hi
 
 int OE_FN(refresh_elements_per_block_)() { return refresh_epb<OE_GATHER_Q>(); }
 
-void OE_FN(launch_elem_tasks_)(const ElemTaskArgs& a, hipStream_t s) {
+#if OE_GATHER_S != 27
+namespace {
+// The element type's shape-derivative table as float, the values the scenes
+// upload as their dN (FemConstants), built once.
+const ShapeGradArg<OE_GATHER_S, OE_GATHER_Q>& shape_grad_arg() {
+    constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;
+    static const ShapeGradArg<S, Q> g = [] {
+        ShapeGradArg<S, Q> t{};
+        if constexpr (grad_by_value<S, Q>()) {
+            const Element el = S == 4 ? Tetra : S == 8 ? Hexa : S == 10 ? Tetra10 : S == 20 ? Tetra20 : Hexa27;
+            const ShapeTable& st = shape_table(el);
+            if (static_cast<int>(st.dN.size()) != Q * S * 3) throw std::logic_error("shape table size mismatch");
+            for (int i = 0; i < Q * S * 3; ++i) t.dN[i] = static_cast<float>(st.dN[i]);
+        }
+        return t;
+    }();
+    return g;
+}
+}  // namespace
+
+void OE_FN(launch_fused_tasks_)(const FusedTaskArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
-    hipLaunchKernelGGL((k_elem_tasks<OE_GATHER_S, OE_GATHER_Q>),
-                       dim3((a.count + kElemTasksPerBlock - 1) / kElemTasksPerBlock), dim3(kElemTasksPerBlock), 0, s,
-                       a);
+    hipLaunchKernelGGL((k_fused_tasks<OE_GATHER_S, OE_GATHER_Q>), dim3(a.count), dim3(kFusedThreads), 0, s, a,
+                       shape_grad_arg());
 }
 
-void OE_FN(launch_batch_elem_tasks_)(const BatchElemTaskArgs& b, int nblocks, hipStream_t s) {
+void OE_FN(launch_batch_fused_tasks_)(const BatchFusedArgs& b, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL((k_batch_elem_tasks<OE_GATHER_S, OE_GATHER_Q>), dim3(nblocks), dim3(kElemTasksPerBlock), 0, s,
-                       b);
+    hipLaunchKernelGGL((k_batch_fused_tasks<OE_GATHER_S, OE_GATHER_Q>), dim3(nblocks), dim3(kFusedThreads), 0, s, b,
+                       shape_grad_arg());
 }
+
+#endif
 
 void OE_FN(launch_batch_ju_refresh_)(const BatchRefreshArgs& b, int nblocks, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;

@@ -46,6 +46,7 @@ struct SceneDev {
     float4* y;
     float* ju;        // cached displacement gradients, or null
     float* partial;   // element-task partials (9 floats per owners entry), or null
+    const int4* fblocks;   // vertex blocks of the fused element tasks, or null
     float damping;
     int iterations;
     float g[3];
@@ -77,27 +78,41 @@ struct BatchGatherArgs {
     float dt;
 };
 
-// Element tasks for the task form of a VBD colour pass: one lane per owners
-// entry (element e, local node r) of the colour, all Q points of e, partial
-// force / Hessian of node r (9 floats) written at the owners index.
-struct ElemTaskArgs {
+// Fused element tasks: the task form of a VBD colour pass in one kernel for
+// element types other than the 27-node hexahedron.  A block takes a run of
+// consecutive slots of one colour (a "vertex block", int4 {first slot, slot
+// count, first owners index, incidence count}) whose incidences fit
+// kFusedIncPerBlock.  Its threads first take one incidence (element e, local
+// node r) each, kFusedThreads apart, and put the force / Hessian partial of
+// node r over all Q points of e in LDS; then one thread per slot adds its
+// vertex's partials in the owners order and runs the local solve.
+constexpr int kFusedThreads = 256;
+constexpr int kFusedRuns = 2;
+constexpr int kFusedIncPerBlock = kFusedThreads * kFusedRuns;
+constexpr int kFusedMaxSlots = kFusedThreads;
+
+struct FusedTaskArgs {
     PSView ps;
     FemView fem;
+    OwnersView own;
     MaterialView mt;
-    const int* own;
-    float* partial;
-    int first;   // first owners index of the colour
-    int count;   // incidences of the colour
+    const float4* y;
+    const int4* blocks;   // vertex blocks of the scene, colour-major
+    float damping;
+    float dt;
+    int first_block;      // the colour's vertex blocks: [first_block, first_block + count)
+    int count;
 };
-// Batched: BatchSeg.first / count are an owners range of one scene.
-struct BatchElemTaskArgs {
+// Batched: BatchSeg.first / count are a range of one scene's vertex blocks,
+// one block each.
+struct BatchFusedArgs {
     const SceneDev* scenes;
     const BatchSeg* segs;
     int seg_first;
     int seg_count;
     int iteration;
+    float dt;
 };
-constexpr int kElemTasksPerBlock = 256;
 
 // Batched refresh of the cached displacement gradients: VSeg.count is the
 // scene's element count; each block takes refresh_elements_per_block(S, Q)
@@ -117,8 +132,8 @@ struct BatchRefreshArgs {
     void launch_ju_refresh_##S##_##Q(const PSView& ps, const FemView& f, float* ju, hipStream_t s);          \
     void launch_batch_ju_refresh_##S##_##Q(const BatchRefreshArgs& b, int nblocks, hipStream_t s);          \
     int refresh_elements_per_block_##S##_##Q();                                                               \
-    void launch_elem_tasks_##S##_##Q(const ElemTaskArgs& a, hipStream_t s);                                   \
-    void launch_batch_elem_tasks_##S##_##Q(const BatchElemTaskArgs& b, int nblocks, hipStream_t s);
+    void launch_fused_tasks_##S##_##Q(const FusedTaskArgs& a, hipStream_t s);                                 \
+    void launch_batch_fused_tasks_##S##_##Q(const BatchFusedArgs& b, int nblocks, hipStream_t s);
 OE_GATHER_DECL(4, 1)
 OE_GATHER_DECL(8, 8)
 OE_GATHER_DECL(10, 4)
@@ -146,22 +161,20 @@ inline bool launch_ju_refresh(int S, int Q, const PSView& ps, const FemView& f, 
     return true;
 }
 
-inline bool launch_elem_tasks(int S, int Q, const ElemTaskArgs& a, hipStream_t s) {
-    if (S == 4 && Q == 1) launch_elem_tasks_4_1(a, s);
-    else if (S == 8 && Q == 8) launch_elem_tasks_8_8(a, s);
-    else if (S == 10 && Q == 4) launch_elem_tasks_10_4(a, s);
-    else if (S == 20 && Q == 11) launch_elem_tasks_20_11(a, s);
-    else if (S == 27 && Q == 27) launch_elem_tasks_27_27(a, s);
+inline bool launch_fused_tasks(int S, int Q, const FusedTaskArgs& a, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_fused_tasks_4_1(a, s);
+    else if (S == 8 && Q == 8) launch_fused_tasks_8_8(a, s);
+    else if (S == 10 && Q == 4) launch_fused_tasks_10_4(a, s);
+    else if (S == 20 && Q == 11) launch_fused_tasks_20_11(a, s);
     else return false;
     return true;
 }
 
-inline bool launch_batch_elem_tasks(int S, int Q, const BatchElemTaskArgs& b, int nblocks, hipStream_t s) {
-    if (S == 4 && Q == 1) launch_batch_elem_tasks_4_1(b, nblocks, s);
-    else if (S == 8 && Q == 8) launch_batch_elem_tasks_8_8(b, nblocks, s);
-    else if (S == 10 && Q == 4) launch_batch_elem_tasks_10_4(b, nblocks, s);
-    else if (S == 20 && Q == 11) launch_batch_elem_tasks_20_11(b, nblocks, s);
-    else if (S == 27 && Q == 27) launch_batch_elem_tasks_27_27(b, nblocks, s);
+inline bool launch_batch_fused_tasks(int S, int Q, const BatchFusedArgs& b, int nblocks, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_batch_fused_tasks_4_1(b, nblocks, s);
+    else if (S == 8 && Q == 8) launch_batch_fused_tasks_8_8(b, nblocks, s);
+    else if (S == 10 && Q == 4) launch_batch_fused_tasks_10_4(b, nblocks, s);
+    else if (S == 20 && Q == 11) launch_batch_fused_tasks_20_11(b, nblocks, s);
     else return false;
     return true;
 }

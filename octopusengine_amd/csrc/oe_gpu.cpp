@@ -238,7 +238,7 @@ GatherTile default_tile(Element e) {
     switch (e) {
         case Tetra: return {8, 1, PASS_GATHER};
         case Hexa: return {16, 1, PASS_GATHER};
-        case Tetra10: return {16, 2, PASS_GATHER};
+        case Tetra10: return {64, 1, PASS_TASKS};
         case Tetra20: return {64, 1, PASS_GATHER};
         case Hexa27: return {64, 1, PASS_TASKS};
         default: return {16, 1, PASS_GATHER};
@@ -267,22 +267,42 @@ GPU_VBD_FEM::GPU_VBD_FEM(Engine* e, Element el, const HostMesh& mesh, Material m
     conflicts = count_conflicts(g, colors);
     if (conflicts != 0 && version != Better_Coloration)
         throw std::logic_error("DSAT produced a conflicting colouring");
-    owners = build_owners(mesh, colors);
+    owners = build_owners(mesh, colors, true);
     upload_owners(eng, owners, d_slot_vid, d_slot_nbo, d_own);
     if (version == Better_Coloration) p_new.allocate(&eng->pool, mesh.nb_vertices());
     if (el == Hexa27) has_q2_basis = q2_basis_1d(q2_basis);
     // allocated here, not in step(), which may run under HIP graph capture
     if (cached()) ensure_ju();
-    if (tasks()) ensure_partial();
+    if (tasks()) {
+        try {
+            prepare_tasks();
+        } catch (const std::invalid_argument&) {
+            // a vertex with more incidences than a fused task block holds: the
+            // default pass falls back to the gather kernel
+            if (element == Hexa27) throw;
+            tile = GatherTile{16, 2, PASS_GATHER};
+        }
+    }
 }
 
 GPU_VBD_FEM::~GPU_VBD_FEM() {
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
 }
 
-float* GPU_VBD_FEM::ensure_partial() {
-    if (!partial.get()) partial.allocate(&eng->pool, owners.own.size() * 9);
-    return partial.get();
+void GPU_VBD_FEM::prepare_tasks() {
+    if (element == Hexa27) {
+        if (!partial.get()) partial.allocate(&eng->pool, owners.own.size() * 9);
+        return;
+    }
+    if (d_fblk.get()) return;
+    std::vector<int32_t> blocks;
+    std::vector<int32_t> begin;
+    build_vertex_blocks(owners, kFusedIncPerBlock, kFusedMaxSlots, blocks, begin);
+    std::vector<int4> b4(blocks.size() / 4);
+    for (size_t i = 0; i < b4.size(); ++i)
+        b4[i] = make_int4(blocks[4 * i], blocks[4 * i + 1], blocks[4 * i + 2], blocks[4 * i + 3]);
+    d_fblk = DBuf<int4>(&eng->pool, b4);
+    fblk_color_begin = std::move(begin);
 }
 
 float* GPU_VBD_FEM::ensure_ju() {
@@ -354,7 +374,8 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
         if (!launch_ju_refresh(S, Q, a.ps, a.fem, a.ju, s)) throw std::invalid_argument("unsupported element");
     }
     const bool by_tasks = tasks();
-    if (by_tasks && !partial.get()) throw std::logic_error("task partials not allocated");
+    if (by_tasks && !(element == Hexa27 ? partial.get() != nullptr : d_fblk.get() != nullptr))
+        throw std::logic_error("element-task buffers not prepared");
     for (int c : order) {
         a.first_slot = owners.color_begin[c];
         a.count = owners.color_begin[c + 1] - a.first_slot;
@@ -366,18 +387,17 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
             }
             OE_HIP(hipEventRecord(ev_pool[ev_used], s));
         }
-        if (by_tasks) {
+        if (by_tasks && element == Hexa27) {
             const int first_own = a.count > 0 ? owners.slot_off[a.first_slot] : 0;
             const int ntask = static_cast<int>(owners.color_incidences[c]);
-            if (element == Hexa27) {
-                launch_q2_tasks(Q2TaskArgs{a.ps, a.fem, a.mt, q2_basis, d_own.get(), partial.get(), first_own, ntask},
-                                s);
-            } else if (!launch_elem_tasks(S, Q, ElemTaskArgs{a.ps, a.fem, a.mt, d_own.get(), partial.get(), first_own,
-                                                             ntask},
-                                          s)) {
-                throw std::invalid_argument("unsupported element");
-            }
+            launch_q2_tasks(Q2TaskArgs{a.ps, a.fem, a.mt, q2_basis, d_own.get(), partial.get(), first_own, ntask}, s);
             launch_vertex_solve(VertexSolveArgs{a.ps, a.own, y, partial.get(), damping, dt, a.first_slot, a.count}, s);
+        } else if (by_tasks) {
+            const int b0 = fblk_color_begin[c];
+            if (!launch_fused_tasks(S, Q, FusedTaskArgs{a.ps, a.fem, a.own, a.mt, y, d_fblk.get(), damping, dt, b0,
+                                                        fblk_color_begin[c + 1] - b0},
+                                    s))
+                throw std::invalid_argument("unsupported element");
         } else if (!launch_gather(S, Q, tile.L, tile.QL, a, s)) {
             throw std::invalid_argument("unsupported gather tile shape");
         }

@@ -138,7 +138,18 @@ struct OwnersCSR {
     std::vector<int32_t> color_max_nb;  // max incident elements per colour
     std::vector<int64_t> color_incidences;  // sum of incident elements per colour
 };
-OwnersCSR build_owners(const HostMesh& m, const Coloration& col);
+// by_valence: within each colour, vertices with more incident elements first
+// (stable, ascending id otherwise); else ascending id.
+OwnersCSR build_owners(const HostMesh& m, const Coloration& col, bool by_valence = false);
+
+// Vertex blocks of the fused element-task pass: per colour, runs of
+// consecutive slots with at most max_inc incidences and max_slots slots in
+// total, 4 ints per block {first slot, slot count, first owners index,
+// incidence count}, colour-major; color_begin[c] = first block of colour c.
+// Throws std::invalid_argument when one vertex alone has more than max_inc
+// incidences.
+void build_vertex_blocks(const OwnersCSR& o, int max_inc, int max_slots, std::vector<int32_t>& blocks,
+                         std::vector<int32_t>& color_begin);
 
 // Element colour schedule for XPBD (reference src/GPU/PBD/GPU_PBD_FEM.cu:528-553).
 struct ElementSchedule {

@@ -66,14 +66,15 @@ __device__ __forceinline__ void group_sum(Acc9& a) {
     for (int i = 0; i < 9; ++i) a.v[i] = group_sum<L>(a.v[i]);
 }
 
-// Jx = sum_k x_k (x) dN_k  (column-major: Jx[c*3+r] = sum_k x_k[r] dN_k[c])
+// Jx = sum_k x_k (x) dN_k  (column-major: Jx[c*3+r] = sum_k x_k[r] dN_k[c]);
+// dn points at the element's shape-derivative table in device memory
 template <int S>
 __device__ __forceinline__ void jx_from_nodes(const float (&xs)[S][3], const float* __restrict__ dn, float Jx[9]) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) Jx[i] = 0.f;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        const float d0 = dn[3 * k], d1 = dn[3 * k + 1], d2 = dn[3 * k + 2];
+        const float d0 = gload(dn, 3 * k), d1 = gload(dn, 3 * k + 1), d2 = gload(dn, 3 * k + 2);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             Jx[0 + r] = fmaf(xs[k][r], d0, Jx[0 + r]);

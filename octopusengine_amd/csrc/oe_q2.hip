@@ -26,12 +26,14 @@ namespace oe {
 
 namespace {
 
-constexpr int kTasksPerBlock = 2;   // 64-thread blocks, one task per 32-lane half
+constexpr int kTasksPerHalf = 4;                     // tasks a 32-lane half works through in turn
+constexpr int kTasksPerBlock = 2 * kTasksPerHalf;    // 64-thread blocks (one wave, two halves)
 
-// tensor place (a * 9 + b * 3 + c) of node n, and node of tensor place t
+// tensor place (a * 9 + b * 3 + c) of node n, and node of tensor place t;
+// entries 27-31 repeat entry 26 for the idle lanes
 struct Q2Tables {
-    int place[27];
-    int node[27];
+    int place[32];
+    int node[32];
 };
 constexpr Q2Tables make_tables() {
     Q2Tables t{};
@@ -39,121 +41,196 @@ constexpr Q2Tables make_tables() {
         t.place[n] = q2::kNodeABC[n];
         t.node[q2::kNodeABC[n]] = n;
     }
+    for (int n = 27; n < 32; ++n) {
+        t.place[n] = t.place[26];
+        t.node[n] = t.node[26];
+    }
     return t;
 }
 __constant__ Q2Tables kQ2 = make_tables();
 
-__device__ __forceinline__ float sel3(float v0, float v1, float v2, int k) {
-    return k == 0 ? v0 : (k == 1 ? v1 : v2);
+// The three 1D factors and their derivatives at one Gauss coordinate.
+struct Basis3 {
+    float l0, l1, l2, d0, d1, d2;
+};
+__device__ __forceinline__ Basis3 basis_at(const q2::Basis1D& B, int i) {
+    const bool i0 = i == 0, i1 = i == 1;
+    return Basis3{i0 ? B.L[0][0] : (i1 ? B.L[0][1] : B.L[0][2]), i0 ? B.L[1][0] : (i1 ? B.L[1][1] : B.L[1][2]),
+                  i0 ? B.L[2][0] : (i1 ? B.L[2][1] : B.L[2][2]), i0 ? B.D[0][0] : (i1 ? B.D[0][1] : B.D[0][2]),
+                  i0 ? B.D[1][0] : (i1 ? B.D[1][1] : B.D[1][2]), i0 ? B.D[2][0] : (i1 ? B.D[2][1] : B.D[2][2])};
 }
 
+// One half's LDS.  Rows 27-31 are written by the idle lanes and never read.
+// x reuses u's space: u is dead once stage 1 has read it, x once stage 3 has.
 struct TaskLDS {
-    float u[27][3];    // tensor place -> displacement
-    float t[27][6];    // (a, b, k) -> T, Tz
-    float x[27][9];    // (a, j, k) -> X, Xy, Xz
+    float4 t[32][2];   // (a, b, k) -> T, Tz
+    union {
+        float4 u[32];      // tensor place -> displacement
+        float4 x[32][3];   // (a, j, k) -> X, Xy, Xz
+    };
 };
 
-__device__ __forceinline__ void q2_task(const PSView& ps, const FemView& f, const MaterialView& mt,
-                                        const q2::Basis1D& B, const int* __restrict__ own, float* partial, int first,
-                                        int count, int task, TaskLDS& sh) {
-    const int l = threadIdx.x & 31;
-    bool valid = task < count;
-    int eid = 0, r = 0;
-    if (valid) {
-        const int pk = own[first + task];
-        eid = pk >> 5;
-        r = pk & 31;
-        valid = ps.mask[f.topo[static_cast<size_t>(eid) * 27 + r]] != 0;
-    }
-    const bool act = valid && l < 27;
-    // lane l takes tensor indices (p0, p1, p2) = (l / 9, l / 3 % 3, l % 3) in every stage
-    const int p0 = l / 9, p1 = (l / 3) % 3, p2 = l % 3;
+// x + (x at lane ctrl(lane)) for the rows in ROWS; other rows add 0
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_add(float x) {
+    return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
+}
+
+// Sum over each 32-lane half; the half's total ends in its lanes 16-31.
+// Within each 16-lane row: swap neighbours (quad_perm 1,0,3,2), pairs
+// (quad_perm 2,3,0,1), mirror in 8 lanes, mirror in 16 lanes; then row 1
+// (row 3) adds lane 15 (lane 47) of the row before it (row_bcast:15).
+__device__ __forceinline__ float half_sum_to_upper(float x) {
+    x = dpp_add<0xB1>(x);
+    x = dpp_add<0x4E>(x);
+    x = dpp_add<0x141>(x);
+    x = dpp_add<0x140>(x);
+    return dpp_add<0x142, 0xA>(x);
+}
+
+// Lane-constant part of a half's work: its point's tensor indices, the 1D
+// factors of the three stages and its LDS rows.
+struct LanePlan {
+    int l, node, place, q;
+    int ab3, b0, x0;   // stage-1 u rows, stage-2 t rows, stage-3 x rows
+    Basis3 s1, s2, s3;
+};
+__device__ __forceinline__ LanePlan lane_plan(const q2::Basis1D& B) {
+    LanePlan p;
+    p.l = threadIdx.x & 31;
+    const int lc = p.l < 27 ? p.l : 26;
+    const int p0 = lc / 9, p1 = (lc / 3) % 3, p2 = lc % 3;
+    p.node = lc;
+    p.place = kQ2.place[lc];
+    p.q = kQ2.node[lc];
+    p.ab3 = (p0 * 3 + p1) * 3;
+    p.b0 = p0 * 9 + p2;
+    p.x0 = p1 * 3 + p2;
+    p.s1 = basis_at(B, p2);
+    p.s2 = basis_at(B, p1);
+    p.s3 = basis_at(B, p0);
+    return p;
+}
+
+// One task: incidence (eid, r), the lane's node displacement u already loaded.
+// Returns the lane's contribution to node r (zero on the idle lanes).
+__device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialView& mt, const LanePlan& P, int eid,
+                                             int r, float4 u, TaskLDS& sh, Acc9& acc) {
+    // the point's element record and dN_r, loaded while the stages run
+    float JXi[9], V;
+    load_element_record(f, eid, P.q, JXi, V);
+    if (P.l >= 27) V = 0.f;
+    const float dnr[3] = {gload(f.dN, (P.q * 27 + r) * 3), gload(f.dN, (P.q * 27 + r) * 3 + 1),
+                          gload(f.dN, (P.q * 27 + r) * 3 + 2)};
     // stage 0
-    if (act) {
-        const float4 u = ps.p[f.topo[static_cast<size_t>(eid) * 27 + l]];
-        const int t = kQ2.place[l];
-        sh.u[t][0] = u.x;
-        sh.u[t][1] = u.y;
-        sh.u[t][2] = u.z;
+    sh.u[P.place] = u;
+    __syncthreads();
+    // stage 1: (a, b, k)
+    {
+        const float4 u0 = sh.u[P.ab3], u1 = sh.u[P.ab3 + 1], u2 = sh.u[P.ab3 + 2];
+        const Basis3& b = P.s1;
+        sh.t[P.l][0] = make_float4(b.l0 * u0.x + b.l1 * u1.x + b.l2 * u2.x, b.l0 * u0.y + b.l1 * u1.y + b.l2 * u2.y,
+                                   b.l0 * u0.z + b.l1 * u1.z + b.l2 * u2.z, 0.f);
+        sh.t[P.l][1] = make_float4(b.d0 * u0.x + b.d1 * u1.x + b.d2 * u2.x, b.d0 * u0.y + b.d1 * u1.y + b.d2 * u2.y,
+                                   b.d0 * u0.z + b.d1 * u1.z + b.d2 * u2.z, 0.f);
     }
     __syncthreads();
-    // stage 1: (a, b, k) = (p0, p1, p2)
-    if (act) {
-        const float l0 = sel3(B.L[0][0], B.L[0][1], B.L[0][2], p2), l1 = sel3(B.L[1][0], B.L[1][1], B.L[1][2], p2),
-                    l2 = sel3(B.L[2][0], B.L[2][1], B.L[2][2], p2);
-        const float d0 = sel3(B.D[0][0], B.D[0][1], B.D[0][2], p2), d1 = sel3(B.D[1][0], B.D[1][1], B.D[1][2], p2),
-                    d2 = sel3(B.D[2][0], B.D[2][1], B.D[2][2], p2);
-        const int ab = p0 * 3 + p1;
-#pragma unroll
-        for (int comp = 0; comp < 3; ++comp) {
-            const float u0 = sh.u[ab * 3][comp], u1 = sh.u[ab * 3 + 1][comp], u2 = sh.u[ab * 3 + 2][comp];
-            sh.t[l][comp] = l0 * u0 + l1 * u1 + l2 * u2;
-            sh.t[l][3 + comp] = d0 * u0 + d1 * u1 + d2 * u2;
-        }
+    // stage 2: (a, j, k); T of (a, b, k) sits at row a * 9 + b * 3 + k
+    {
+        const float4 t0 = sh.t[P.b0][0], t1 = sh.t[P.b0 + 3][0], t2 = sh.t[P.b0 + 6][0];
+        const float4 z0 = sh.t[P.b0][1], z1 = sh.t[P.b0 + 3][1], z2 = sh.t[P.b0 + 6][1];
+        const Basis3& b = P.s2;
+        sh.x[P.l][0] = make_float4(b.l0 * t0.x + b.l1 * t1.x + b.l2 * t2.x, b.l0 * t0.y + b.l1 * t1.y + b.l2 * t2.y,
+                                   b.l0 * t0.z + b.l1 * t1.z + b.l2 * t2.z, 0.f);
+        sh.x[P.l][1] = make_float4(b.d0 * t0.x + b.d1 * t1.x + b.d2 * t2.x, b.d0 * t0.y + b.d1 * t1.y + b.d2 * t2.y,
+                                   b.d0 * t0.z + b.d1 * t1.z + b.d2 * t2.z, 0.f);
+        sh.x[P.l][2] = make_float4(b.l0 * z0.x + b.l1 * z1.x + b.l2 * z2.x, b.l0 * z0.y + b.l1 * z1.y + b.l2 * z2.y,
+                                   b.l0 * z0.z + b.l1 * z1.z + b.l2 * z2.z, 0.f);
     }
     __syncthreads();
-    // stage 2: (a, j, k) = (p0, p1, p2); T of (a, b, k) sits at lane a * 9 + b * 3 + k
-    if (act) {
-        const float l0 = sel3(B.L[0][0], B.L[0][1], B.L[0][2], p1), l1 = sel3(B.L[1][0], B.L[1][1], B.L[1][2], p1),
-                    l2 = sel3(B.L[2][0], B.L[2][1], B.L[2][2], p1);
-        const float d0 = sel3(B.D[0][0], B.D[0][1], B.D[0][2], p1), d1 = sel3(B.D[1][0], B.D[1][1], B.D[1][2], p1),
-                    d2 = sel3(B.D[2][0], B.D[2][1], B.D[2][2], p1);
-        const int b0 = p0 * 9 + p2;
-#pragma unroll
-        for (int comp = 0; comp < 3; ++comp) {
-            const float t0 = sh.t[b0][comp], t1 = sh.t[b0 + 3][comp], t2 = sh.t[b0 + 6][comp];
-            const float z0 = sh.t[b0][3 + comp], z1 = sh.t[b0 + 3][3 + comp], z2 = sh.t[b0 + 6][3 + comp];
-            sh.x[l][comp] = l0 * t0 + l1 * t1 + l2 * t2;
-            sh.x[l][3 + comp] = d0 * t0 + d1 * t1 + d2 * t2;
-            sh.x[l][6 + comp] = l0 * z0 + l1 * z1 + l2 * z2;
-        }
+    // stage 3: point (i, j, k); X of (a, j, k) sits at row a * 9 + j * 3 + k
+    float Ju[9];   // column-major: Ju[c * 3 + comp] = d u_comp / d xi_c
+    {
+        const float4 xa = sh.x[P.x0][0], xb = sh.x[P.x0 + 9][0], xc = sh.x[P.x0 + 18][0];
+        const float4 ya = sh.x[P.x0][1], yb = sh.x[P.x0 + 9][1], yc = sh.x[P.x0 + 18][1];
+        const float4 za = sh.x[P.x0][2], zb = sh.x[P.x0 + 9][2], zc = sh.x[P.x0 + 18][2];
+        const Basis3& b = P.s3;
+        Ju[0] = b.d0 * xa.x + b.d1 * xb.x + b.d2 * xc.x;
+        Ju[1] = b.d0 * xa.y + b.d1 * xb.y + b.d2 * xc.y;
+        Ju[2] = b.d0 * xa.z + b.d1 * xb.z + b.d2 * xc.z;
+        Ju[3] = b.l0 * ya.x + b.l1 * yb.x + b.l2 * yc.x;
+        Ju[4] = b.l0 * ya.y + b.l1 * yb.y + b.l2 * yc.y;
+        Ju[5] = b.l0 * ya.z + b.l1 * yb.z + b.l2 * yc.z;
+        Ju[6] = b.l0 * za.x + b.l1 * zb.x + b.l2 * zc.x;
+        Ju[7] = b.l0 * za.y + b.l1 * zb.y + b.l2 * zc.y;
+        Ju[8] = b.l0 * za.z + b.l1 * zb.z + b.l2 * zc.z;
     }
+    // stage 3 has read x before the next task's stage 0 overwrites it (u shares x's space)
     __syncthreads();
-    // stage 3: point (i, j, k) = (p0, p1, p2); X of (a, j, k) sits at lane a * 9 + j * 3 + k
-    Acc9 acc;
-    acc_zero(acc);
-    if (act) {
-        const float l0 = sel3(B.L[0][0], B.L[0][1], B.L[0][2], p0), l1 = sel3(B.L[1][0], B.L[1][1], B.L[1][2], p0),
-                    l2 = sel3(B.L[2][0], B.L[2][1], B.L[2][2], p0);
-        const float d0 = sel3(B.D[0][0], B.D[0][1], B.D[0][2], p0), d1 = sel3(B.D[1][0], B.D[1][1], B.D[1][2], p0),
-                    d2 = sel3(B.D[2][0], B.D[2][1], B.D[2][2], p0);
-        const int x0 = p1 * 3 + p2;
-        float Ju[9];   // column-major: Ju[c * 3 + comp] = d u_comp / d xi_c
+    nh_contrib(Ju, JXi, V, dnr, mt, acc);
+}
+
+// A half's run of tasks [task0, task0 + kTasksPerHalf) of one colour.  The
+// loads run ahead of the work: task t's displacement, task t+1's node index
+// and task t+2's owners entry are issued before task t's stages.  Indices past
+// the colour's last task are clamped to it (that work is done and not stored).
+__device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, const MaterialView& mt,
+                                           const LanePlan& P, const int* __restrict__ own,
+                                           float* __restrict__ partial, int first, int count, int task0,
+                                           TaskLDS& sh) {
+    const int last = count - 1;
+    int pk = gload(own, first + min(task0, last));
+    int pk_next = gload(own, first + min(task0 + 1, last));
+    int vid = gload(f.topo, static_cast<size_t>(pk >> 5) * 27 + P.node);
+#pragma unroll 1
+    for (int it = 0; it < kTasksPerHalf; ++it) {
+        const int task = task0 + it;
+        const float4 u = gload(static_cast<const float4*>(ps.p), vid);
+        const int eid = pk >> 5, r = pk & 31;
+        pk = pk_next;
+        vid = gload(f.topo, static_cast<size_t>(pk >> 5) * 27 + P.node);
+        pk_next = gload(own, first + min(task + 2, last));
+        Acc9 acc;
+        acc_zero(acc);
+        q2_task_body(f, mt, P, eid, r, u, sh, acc);
 #pragma unroll
-        for (int comp = 0; comp < 3; ++comp) {
-            Ju[comp] = d0 * sh.x[x0][comp] + d1 * sh.x[x0 + 9][comp] + d2 * sh.x[x0 + 18][comp];
-            Ju[3 + comp] = l0 * sh.x[x0][3 + comp] + l1 * sh.x[x0 + 9][3 + comp] + l2 * sh.x[x0 + 18][3 + comp];
-            Ju[6 + comp] = l0 * sh.x[x0][6 + comp] + l1 * sh.x[x0 + 9][6 + comp] + l2 * sh.x[x0 + 18][6 + comp];
+        for (int i = 0; i < 9; ++i) acc.v[i] = half_sum_to_upper(acc.v[i]);
+        // lanes 16-24 of the half write the task's partial, one float each
+        const int i = P.l - 16;
+        if (task < count && i >= 0 && i < 9) {
+            float v = acc.v[0];
+#pragma unroll
+            for (int k = 1; k < 9; ++k) v = i == k ? acc.v[k] : v;
+            gstore(partial, static_cast<size_t>(first + task) * 9 + i, v);
         }
-        const int q = kQ2.node[l];
-        float JXi[9], V;
-        load_element_record(f, eid, q, JXi, V);
-        nh_contrib(Ju, JXi, V, f.dN + (q * 27 + r) * 3, mt, acc);
-    }
-    group_sum<32>(acc);
-    if (valid && l == 0) {
-        float* out = partial + static_cast<size_t>(first + task) * 9;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) out[i] = acc.v[i];
     }
 }
 
 __global__ __launch_bounds__(64) void k_q2_tasks(Q2TaskArgs a) {
-    __shared__ TaskLDS sh[kTasksPerBlock];
+    __shared__ TaskLDS sh[2];
     const int half = threadIdx.x >> 5;
-    const int task = xcd_block(blockIdx.x, gridDim.x) * kTasksPerBlock + half;
-    q2_task(a.ps, a.fem, a.mt, a.basis, a.own, a.partial, a.first, a.count, task, sh[half]);
+    const int task0 = (xcd_block(blockIdx.x, gridDim.x) * 2 + half) * kTasksPerHalf;
+    const LanePlan P = lane_plan(a.basis);
+    q2_task_run(a.ps, a.fem, a.mt, P, a.own, a.partial, a.first, a.count, task0, sh[half]);
 }
 
 __global__ __launch_bounds__(64) void k_batch_q2_tasks(BatchQ2TaskArgs b) {
-    __shared__ TaskLDS sh[kTasksPerBlock];
+    __shared__ TaskLDS sh[2];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
     const SceneDev& sd = b.scenes[seg.scene];
     if (b.iteration >= sd.iterations) return;     // uniform over the block
+    // the scene's views, read once before the first partial store (which the
+    // compiler could not otherwise tell apart from the scene table)
+    const PSView ps = sd.ps;
+    const FemView fem = sd.fem;
+    const MaterialView mt = sd.mt;
+    const int* own = sd.own.own;
+    float* partial = sd.partial;
     const int half = threadIdx.x >> 5;
-    const int task = (blk - seg.block_begin) * kTasksPerBlock + half;
-    q2_task(sd.ps, sd.fem, sd.mt, b.basis, sd.own.own, sd.partial, seg.first, seg.count, task, sh[half]);
+    const int task0 = ((blk - seg.block_begin) * 2 + half) * kTasksPerHalf;
+    const LanePlan P = lane_plan(b.basis);
+    q2_task_run(ps, fem, mt, P, own, partial, seg.first, seg.count, task0, sh[half]);
 }
 
 // Sum of a vertex's partials (owners order) and the local solve.
