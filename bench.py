@@ -185,8 +185,8 @@ def batched_leg(args, eng) -> dict:
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_detail": traffic,
-                     "kernel": "VBD colour pass over all scenes: k_batch_gather (P2), "
-                               "k_batch_q2_tasks + k_batch_vertex_solve (Q2)",
+                     "kernel": "VBD colour pass over all scenes: k_batch_fused_tasks<10,4> (P2), "
+                               "k_batch_q2_fused (Q2)",
                      "bytes_per_launch": bytes_total / max(launches, 1), "avg_launch_us": avg_launch_s * 1e6,
                      "launches": launches},
         "_scenes": scenes,

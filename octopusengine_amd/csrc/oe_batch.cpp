@@ -62,11 +62,9 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
     if (t.mode == PASS_TASKS && s.mesh.element == Hexa27 && !s.vbd_fem->has_q2_basis)
         throw std::invalid_argument("no tensor factors for this 27-node hexahedron");
     d.ju = t.mode == PASS_CACHE ? s.vbd_fem->ensure_ju() : nullptr;
-    d.partial = nullptr;
     d.fblocks = nullptr;
     if (t.mode == PASS_TASKS) {
         s.vbd_fem->prepare_tasks();
-        d.partial = s.vbd_fem->partial.get();
         d.fblocks = s.vbd_fem->d_fblk.get();
     }
     d.ps = s.ps->get_parameters();
@@ -100,7 +98,6 @@ void Schedule::clear() {
     glaunches.clear();
     vsegs.clear();
     vblocks = 0;
-    tsegs.clear();
     jlaunches.clear();
     jsegs.clear();
     max_iterations = 0;
@@ -126,7 +123,6 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
             GLaunch gl{};
             gl.color = c;
             gl.seg_first = static_cast<int>(segs.size());
-            gl.tseg_first = static_cast<int>(tsegs.size());
             bool any = false;
             for (size_t i = 0; i < scenes.size(); ++i) {
                 Scene* s = scenes[i];
@@ -144,7 +140,7 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     any = true;
                     if (gl.mode == PASS_TASKS) q2_basis = s->vbd_fem->q2_basis;
                 }
-                if (gl.mode == PASS_TASKS && el != Hexa27) {
+                if (gl.mode == PASS_TASKS) {
                     // fused tasks: one block per vertex block of the colour
                     const std::vector<int32_t>& fb = s->vbd_fem->fblk_color_begin;
                     if (fb.empty()) throw std::logic_error("vertex blocks not prepared");
@@ -152,26 +148,15 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     segs.push_back(BatchSeg{table_index[i], fb[c], nfb, gl.nblocks});
                     gl.nblocks += nfb;
                 } else {
-                    // gather: L lanes per slot; Q2 tasks: the vertex solve takes one thread per slot
-                    const int lanes = gl.mode == PASS_TASKS ? 1 : gl.L;
-                    const int nb = static_cast<int>((static_cast<int64_t>(count) * lanes + kBlock - 1) / kBlock);
+                    // gather: L lanes per slot
+                    const int nb = static_cast<int>((static_cast<int64_t>(count) * gl.L + kBlock - 1) / kBlock);
                     segs.push_back(BatchSeg{table_index[i], first, count, gl.nblocks});
                     gl.nblocks += nb;
-                }
-                if (gl.mode == PASS_TASKS && el == Hexa27) {
-                    const int tasks = static_cast<int>(s->vbd_fem->owners.color_incidences[c]);
-                    const int tpb = q2_tasks_per_block();
-                    if (tasks > 0) {
-                        tsegs.push_back(BatchSeg{table_index[i], s->vbd_fem->owners.slot_off[first], tasks,
-                                                 gl.tnblocks});
-                        gl.tnblocks += (tasks + tpb - 1) / tpb;
-                    }
                 }
                 gl.slots += count;
                 gl.incidences += s->vbd_fem->owners.color_incidences[c];
             }
             gl.seg_count = static_cast<int>(segs.size()) - gl.seg_first;
-            gl.tseg_count = static_cast<int>(tsegs.size()) - gl.tseg_first;
             if (any) glaunches.push_back(gl);
         }
     }
@@ -217,7 +202,6 @@ void DeviceSchedule::upload(Engine* eng, const Schedule& s, hipStream_t stream, 
     upload_grow(eng, segs, s.segs, stream);
     upload_grow(eng, vsegs, s.vsegs, stream);
     upload_grow(eng, jsegs, s.jsegs, stream);
-    upload_grow(eng, tsegs, s.tsegs, stream);
     if (sync) OE_HIP(hipStreamSynchronize(stream));
 }
 
@@ -260,7 +244,7 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
     BatchVArgs v{d_scenes, dev.vsegs.get(), static_cast<int>(sch.vsegs.size()), h, 0};
     BatchGatherArgs g{d_scenes, dev.segs.get(), 0, 0, 0, h};
     BatchRefreshArgs r{d_scenes, dev.jsegs.get(), 0, 0, 0};
-    BatchQ2TaskArgs tk{d_scenes, dev.tsegs.get(), 0, 0, 0, sch.q2_basis};
+    BatchQ2FusedArgs tk{d_scenes, dev.segs.get(), 0, 0, 0, h, sch.q2_basis};
     BatchFusedArgs ft{d_scenes, dev.segs.get(), 0, 0, 0, h};
     for (int sub = 0; sub < substeps; ++sub) {
         launch_batch_predict(v, sch.vblocks, stream);
@@ -277,10 +261,9 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                 g.seg_count = gl.seg_count;
                 if (timer) timer->before(stream);
                 if (gl.mode == PASS_TASKS && gl.S == 27 && gl.Q == 27) {
-                    tk.seg_first = gl.tseg_first;
-                    tk.seg_count = gl.tseg_count;
-                    if (gl.tseg_count > 0) launch_batch_q2_tasks(tk, gl.tnblocks, stream);
-                    launch_batch_vertex_solve(g, gl.nblocks, stream);
+                    tk.seg_first = gl.seg_first;
+                    tk.seg_count = gl.seg_count;
+                    launch_batch_q2_fused(tk, gl.nblocks, stream);
                 } else if (gl.mode == PASS_TASKS) {
                     ft.seg_first = gl.seg_first;
                     ft.seg_count = gl.seg_count;

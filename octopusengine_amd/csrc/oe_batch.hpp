@@ -35,8 +35,7 @@ struct Schedule {
     struct GLaunch {
         int color, S, Q, L, QL;
         int mode;                         // PassMode
-        int seg_first, seg_count, nblocks;      // gather, or vertex solve (PASS_TASKS)
-        int tseg_first, tseg_count, tnblocks;   // element tasks (PASS_TASKS)
+        int seg_first, seg_count, nblocks;   // gather: slot ranges; tasks: vertex-block ranges
         int64_t slots;   // vertex slots covered (all scenes)
         int64_t incidences;
     };
@@ -51,7 +50,6 @@ struct Schedule {
     int vblocks = 0;
     std::vector<JLaunch> jlaunches;
     std::vector<VSeg> jsegs;          // count = the scene's elements
-    std::vector<BatchSeg> tsegs;      // element tasks: owners ranges
     q2::Basis1D q2_basis{};
     int max_iterations = 0;
     int64_t vertex_updates_per_substep = 0;
@@ -68,7 +66,6 @@ struct DeviceSchedule {
     DBuf<BatchSeg> segs;
     DBuf<VSeg> vsegs;
     DBuf<VSeg> jsegs;
-    DBuf<BatchSeg> tsegs;
     void upload(Engine* eng, const Schedule& s, hipStream_t stream, bool sync);
 };
 

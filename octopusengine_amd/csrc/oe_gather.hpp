@@ -45,7 +45,6 @@ struct SceneDev {
     MaterialView mt;
     float4* y;
     float* ju;        // cached displacement gradients, or null
-    float* partial;   // element-task partials (9 floats per owners entry), or null
     const int4* fblocks;   // vertex blocks of the fused element tasks, or null
     float damping;
     int iterations;

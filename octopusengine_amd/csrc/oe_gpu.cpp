@@ -277,10 +277,9 @@ GPU_VBD_FEM::GPU_VBD_FEM(Engine* e, Element el, const HostMesh& mesh, Material m
         try {
             prepare_tasks();
         } catch (const std::invalid_argument&) {
-            // a vertex with more incidences than a fused task block holds: the
+            // a vertex with more incidences than a task block holds: the
             // default pass falls back to the gather kernel
-            if (element == Hexa27) throw;
-            tile = GatherTile{16, 2, PASS_GATHER};
+            tile = element == Hexa27 ? GatherTile{64, 1, PASS_GATHER} : GatherTile{16, 2, PASS_GATHER};
         }
     }
 }
@@ -290,14 +289,13 @@ GPU_VBD_FEM::~GPU_VBD_FEM() {
 }
 
 void GPU_VBD_FEM::prepare_tasks() {
-    if (element == Hexa27) {
-        if (!partial.get()) partial.allocate(&eng->pool, owners.own.size() * 9);
-        return;
-    }
     if (d_fblk.get()) return;
     std::vector<int32_t> blocks;
     std::vector<int32_t> begin;
-    build_vertex_blocks(owners, kFusedIncPerBlock, kFusedMaxSlots, blocks, begin);
+    if (element == Hexa27)
+        build_vertex_blocks(owners, kQ2FusedIncPerBlock, kQ2FusedMaxSlots, blocks, begin);
+    else
+        build_vertex_blocks(owners, kFusedIncPerBlock, kFusedMaxSlots, blocks, begin);
     std::vector<int4> b4(blocks.size() / 4);
     for (size_t i = 0; i < b4.size(); ++i)
         b4[i] = make_int4(blocks[4 * i], blocks[4 * i + 1], blocks[4 * i + 2], blocks[4 * i + 3]);
@@ -374,8 +372,7 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
         if (!launch_ju_refresh(S, Q, a.ps, a.fem, a.ju, s)) throw std::invalid_argument("unsupported element");
     }
     const bool by_tasks = tasks();
-    if (by_tasks && !(element == Hexa27 ? partial.get() != nullptr : d_fblk.get() != nullptr))
-        throw std::logic_error("element-task buffers not prepared");
+    if (by_tasks && !d_fblk.get()) throw std::logic_error("element-task vertex blocks not prepared");
     for (int c : order) {
         a.first_slot = owners.color_begin[c];
         a.count = owners.color_begin[c + 1] - a.first_slot;
@@ -388,10 +385,10 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
             OE_HIP(hipEventRecord(ev_pool[ev_used], s));
         }
         if (by_tasks && element == Hexa27) {
-            const int first_own = a.count > 0 ? owners.slot_off[a.first_slot] : 0;
-            const int ntask = static_cast<int>(owners.color_incidences[c]);
-            launch_q2_tasks(Q2TaskArgs{a.ps, a.fem, a.mt, q2_basis, d_own.get(), partial.get(), first_own, ntask}, s);
-            launch_vertex_solve(VertexSolveArgs{a.ps, a.own, y, partial.get(), damping, dt, a.first_slot, a.count}, s);
+            const int b0 = fblk_color_begin[c];
+            launch_q2_fused(Q2FusedArgs{a.ps, a.fem, a.own, a.mt, q2_basis, y, d_fblk.get(), damping, dt, b0,
+                                        fblk_color_begin[c + 1] - b0},
+                            s);
         } else if (by_tasks) {
             const int b0 = fblk_color_begin[c];
             if (!launch_fused_tasks(S, Q, FusedTaskArgs{a.ps, a.fem, a.own, a.mt, y, d_fblk.get(), damping, dt, b0,

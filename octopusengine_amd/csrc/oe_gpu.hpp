@@ -129,10 +129,10 @@ enum VBD_Version : int { Base = 0, Threaded_Quadrature = 1, Reduction_Symmetry =
 //   gather  one kernel, L lanes per vertex, QL quadrature points per lane;
 //   cache   the same with the displacement gradients of every (element,
 //           quadrature point) kept in device memory between colours (QL = 1);
-//   tasks   element tasks, one per incidence of the colour (L, QL unused):
-//           for 27-node hexahedra sum-factorised gradients and partials in
-//           device memory, then a vertex-solve kernel; for the other types one
-//           fused kernel whose blocks sum the partials in LDS and solve.
+//   tasks   element tasks, one per incidence of the colour (L, QL unused), in
+//           one kernel whose blocks put the partials of a run of the colour's
+//           vertices in LDS, then solve those vertices (sum-factorised
+//           gradients for 27-node hexahedra).
 enum PassMode : int { PASS_GATHER = 0, PASS_CACHE = 1, PASS_TASKS = 2 };
 struct GatherTile {
     int L = 16;
@@ -169,10 +169,9 @@ struct GPU_VBD_FEM final : GPU_FEM {
     void residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r);
     DBuf<float4> d_residual;
     bool cached() const { return tile.mode == PASS_CACHE && version != Better_Coloration; }
-    // element-task buffers, made by prepare_tasks(): for 27-node hexahedra the
-    // partials (9 floats per owners entry), for the other types the vertex
-    // blocks of the fused task kernel
-    DBuf<float> partial;
+    // vertex blocks of the fused element-task kernels (4 ints per block,
+    // colour-major; fblk_color_begin[c] = first block of colour c), made by
+    // prepare_tasks()
     DBuf<int4> d_fblk;
     std::vector<int32_t> fblk_color_begin;
     void prepare_tasks();

@@ -1,11 +1,13 @@
-// VBD colour pass for 27-node hexahedra as two kernels, element tasks then
-// vertex solves, with the displacement gradients of an element's 27
-// quadrature points built by sum factorisation.
+// VBD colour pass for 27-node hexahedra in one kernel: element tasks whose
+// displacement gradients at the 27 quadrature points are built by sum
+// factorisation, then, in the same block, the vertex solves.
 //
-// A task is one incidence (element e, local node r) of a vertex of the colour;
-// the colour's tasks are the contiguous run of the owners array that its
-// vertices own.  A task takes 27 lanes of one 32-lane half of a wave (one
-// lane per quadrature point; lanes 27-31 idle) and works through LDS:
+// A block (256 threads, 8 half-waves) takes a run of consecutive slots of one
+// colour (a vertex block {first slot, slot count, first owners index,
+// incidence count}, at most 32 incidences).  A task is one incidence (element
+// e, local node r); half h takes the block's tasks h, h + 8, h + 16, h + 24.
+// A task uses 27 lanes of its half (one lane per quadrature point; lanes 27-31
+// idle) and works through LDS:
 //   stage 0  lane n loads node n's displacement, stored at its tensor place
 //   stage 1  lane (a, b, k):  T  = sum_c l_c(z_k)  u_abc,  Tz = sum_c l_c'(z_k) u_abc
 //   stage 2  lane (a, j, k):  X  = sum_b l_b(y_j) T_ab,  Xy = sum_b l_b'(y_j) T_ab,
@@ -15,10 +17,10 @@
 // - 72 FMAs per point and coordinate triple instead of the 243 of the plain sum
 // over 27 nodes, one load per node instead of one per (node, point) - then the
 // point's force / Hessian contribution to node r.  The half's lanes add their
-// contributions with shuffles and lane 0 writes the task's 9-float partial at
-// the task's owners index.  The vertex kernel sums a vertex's partials in the
-// owners order and runs the FP64 local solve, the arithmetic of the gather
-// kernel's final lane.
+// contributions with DPP and the task's 9-float partial goes to LDS.  After a
+// block barrier one thread per slot adds its vertex's partials in the owners
+// order and runs the FP64 local solve, the arithmetic of the gather kernel's
+// final lane.
 #include "oe_kernels.hpp"
 #include "oe_q2.hpp"
 
@@ -26,8 +28,21 @@ namespace oe {
 
 namespace {
 
-constexpr int kTasksPerHalf = 4;                     // tasks a 32-lane half works through in turn
-constexpr int kTasksPerBlock = 2 * kTasksPerHalf;    // 64-thread blocks (one wave, two halves)
+constexpr int kHalves = kQ2FusedThreads / 32;            // half-waves per block
+constexpr int kRuns = kQ2FusedIncPerBlock / kHalves;     // tasks a half works through in turn
+static_assert(kRuns * kHalves == kQ2FusedIncPerBlock, "tasks per block");
+static_assert(kQ2FusedMaxSlots <= 64, "phase 2 runs on the first wave");
+
+// The stages exchange data only between the lanes of one half-wave.  A wave's
+// LDS operations are performed in order, so waiting for its own LDS operations
+// (and keeping the compiler from moving memory accesses across) is enough; no
+// workgroup barrier is needed.
+__device__ __forceinline__ void half_sync() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // tensor place (a * 9 + b * 3 + c) of node n, and node of tensor place t;
 // entries 27-31 repeat entry 26 for the idle lanes
@@ -124,7 +139,7 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
                           gload(f.dN, (P.q * 27 + r) * 3 + 2)};
     // stage 0
     sh.u[P.place] = u;
-    __syncthreads();
+    half_sync();
     // stage 1: (a, b, k)
     {
         const float4 u0 = sh.u[P.ab3], u1 = sh.u[P.ab3 + 1], u2 = sh.u[P.ab3 + 2];
@@ -134,7 +149,7 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
         sh.t[P.l][1] = make_float4(b.d0 * u0.x + b.d1 * u1.x + b.d2 * u2.x, b.d0 * u0.y + b.d1 * u1.y + b.d2 * u2.y,
                                    b.d0 * u0.z + b.d1 * u1.z + b.d2 * u2.z, 0.f);
     }
-    __syncthreads();
+    half_sync();
     // stage 2: (a, j, k); T of (a, b, k) sits at row a * 9 + b * 3 + k
     {
         const float4 t0 = sh.t[P.b0][0], t1 = sh.t[P.b0 + 3][0], t2 = sh.t[P.b0 + 6][0];
@@ -147,7 +162,7 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
         sh.x[P.l][2] = make_float4(b.l0 * z0.x + b.l1 * z1.x + b.l2 * z2.x, b.l0 * z0.y + b.l1 * z1.y + b.l2 * z2.y,
                                    b.l0 * z0.z + b.l1 * z1.z + b.l2 * z2.z, 0.f);
     }
-    __syncthreads();
+    half_sync();
     // stage 3: point (i, j, k); X of (a, j, k) sits at row a * 9 + j * 3 + k
     float Ju[9];   // column-major: Ju[c * 3 + comp] = d u_comp / d xi_c
     {
@@ -166,130 +181,112 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
         Ju[8] = b.l0 * za.z + b.l1 * zb.z + b.l2 * zc.z;
     }
     // stage 3 has read x before the next task's stage 0 overwrites it (u shares x's space)
-    __syncthreads();
+    half_sync();
     nh_contrib(Ju, JXi, V, dnr, mt, acc);
 }
 
-// A half's run of tasks [task0, task0 + kTasksPerHalf) of one colour.  The
-// loads run ahead of the work: task t's displacement, task t+1's node index
-// and task t+2's owners entry are issued before task t's stages.  Indices past
-// the colour's last task are clamped to it (that work is done and not stored).
+// A half's tasks task0, task0 + kHalves, ... (fewer than kRuns when the block
+// has fewer incidences).  The loads run ahead of the work: task t's
+// displacement, the next task's node index and the one after's owners entry
+// are issued before task t's stages; indices past the block's last incidence
+// are clamped to it.  Partials go to part[task] (lanes 16-24, one float each).
 __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, const MaterialView& mt,
-                                           const LanePlan& P, const int* __restrict__ own,
-                                           float* __restrict__ partial, int first, int count, int task0,
-                                           TaskLDS& sh) {
-    const int last = count - 1;
-    int pk = gload(own, first + min(task0, last));
-    int pk_next = gload(own, first + min(task0 + 1, last));
+                                           const LanePlan& P, const int* __restrict__ own, int first, int count,
+                                           int task0, TaskLDS& sh, float (*part)[9]) {
+    if (task0 >= count) return;   // uniform over the half
+    const int last = first + count - 1;
+    int pk = gload(own, first + task0);
+    int pk_next = gload(own, min(first + task0 + kHalves, last));
     int vid = gload(f.topo, static_cast<size_t>(pk >> 5) * 27 + P.node);
 #pragma unroll 1
-    for (int it = 0; it < kTasksPerHalf; ++it) {
-        const int task = task0 + it;
+    for (int task = task0; task < count; task += kHalves) {
         const float4 u = gload(static_cast<const float4*>(ps.p), vid);
         const int eid = pk >> 5, r = pk & 31;
         pk = pk_next;
         vid = gload(f.topo, static_cast<size_t>(pk >> 5) * 27 + P.node);
-        pk_next = gload(own, first + min(task + 2, last));
+        pk_next = gload(own, min(first + task + 2 * kHalves, last));
         Acc9 acc;
         acc_zero(acc);
         q2_task_body(f, mt, P, eid, r, u, sh, acc);
 #pragma unroll
         for (int i = 0; i < 9; ++i) acc.v[i] = half_sum_to_upper(acc.v[i]);
-        // lanes 16-24 of the half write the task's partial, one float each
         const int i = P.l - 16;
-        if (task < count && i >= 0 && i < 9) {
+        if (i >= 0 && i < 9) {
             float v = acc.v[0];
 #pragma unroll
             for (int k = 1; k < 9; ++k) v = i == k ? acc.v[k] : v;
-            gstore(partial, static_cast<size_t>(first + task) * 9 + i, v);
+            part[task][i] = v;
         }
     }
 }
 
-__global__ __launch_bounds__(64) void k_q2_tasks(Q2TaskArgs a) {
-    __shared__ TaskLDS sh[2];
-    const int half = threadIdx.x >> 5;
-    const int task0 = (xcd_block(blockIdx.x, gridDim.x) * 2 + half) * kTasksPerHalf;
-    const LanePlan P = lane_plan(a.basis);
-    q2_task_run(a.ps, a.fem, a.mt, P, a.own, a.partial, a.first, a.count, task0, sh[half]);
-}
-
-__global__ __launch_bounds__(64) void k_batch_q2_tasks(BatchQ2TaskArgs b) {
-    __shared__ TaskLDS sh[2];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
-    const SceneDev& sd = b.scenes[seg.scene];
-    if (b.iteration >= sd.iterations) return;     // uniform over the block
-    // the scene's views, read once before the first partial store (which the
-    // compiler could not otherwise tell apart from the scene table)
-    const PSView ps = sd.ps;
-    const FemView fem = sd.fem;
-    const MaterialView mt = sd.mt;
-    const int* own = sd.own.own;
-    float* partial = sd.partial;
-    const int half = threadIdx.x >> 5;
-    const int task0 = ((blk - seg.block_begin) * 2 + half) * kTasksPerHalf;
-    const LanePlan P = lane_plan(b.basis);
-    q2_task_run(ps, fem, mt, P, own, partial, seg.first, seg.count, task0, sh[half]);
-}
-
-// Sum of a vertex's partials (owners order) and the local solve.
-__device__ __forceinline__ void solve_slot(const PSView& ps, const OwnersView& own, const float4* y,
-                                           const float* __restrict__ partial, float damping, float dt, int slot) {
-    const int vid = own.slot_vid[slot];
-    if (ps.mask[vid] == 0) return;
-    const int2 nbo = own.slot_nbo[slot];
+// Phase 2: thread t < slot count adds the partials of slot first_slot + t in
+// the owners order and solves.
+__device__ __forceinline__ void q2_solve_slots(const PSView& ps, const OwnersView& own, const float4* y,
+                                              float damping, float dt, const int4 blk, const float (*part)[9]) {
+    const int t = threadIdx.x;
+    if (t >= blk.y) return;
+    const int slot = blk.x + t;
+    const int vid = gload(own.slot_vid, slot);
+    if (gload(ps.mask, vid) == 0) return;
+    const int2 nbo = gload(own.slot_nbo, slot);
     Acc9 acc;
     acc_zero(acc);
-    for (int j = 0; j < nbo.x; ++j) {
-        const float* p = partial + static_cast<size_t>(nbo.y + j) * 9;
+    for (int j = nbo.y - blk.z, e = j + nbo.x; j < e; ++j) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc.v[i] += p[i];
+        for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
     }
-    const float4 pv = ps.p[vid];
+    const float4 pv = gload(static_cast<const float4*>(ps.p), vid);
     double dx[3];
-    vbd_local_solve(acc, pv, ps.last_p[vid], y[vid], ps.m[vid], damping, dt, dx);
-    ps.p[vid] = make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
-                            pv.z + static_cast<float>(dx[2]), pv.w);
+    vbd_local_solve(acc, pv, gload(static_cast<const float4*>(ps.last_p), vid), gload(y, vid), gload(ps.m, vid),
+                    damping, dt, dx);
+    gstore(ps.p, vid,
+           make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
+                       pv.z + static_cast<float>(dx[2]), pv.w));
 }
 
-__global__ __launch_bounds__(256) void k_vertex_solve(VertexSolveArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.count) return;
-    solve_slot(a.ps, a.own, a.y, a.partial, a.damping, a.dt, a.first_slot + i);
+__device__ __forceinline__ void q2_block(const PSView& ps, const FemView& fem, const OwnersView& own,
+                                         const MaterialView& mt, const q2::Basis1D& basis, const float4* y,
+                                         float damping, float dt, const int4 blk) {
+    __shared__ TaskLDS sh[kHalves];
+    __shared__ float part[kQ2FusedIncPerBlock][9];
+    const int half = threadIdx.x >> 5;
+    const LanePlan P = lane_plan(basis);
+    q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
+    __syncthreads();
+    q2_solve_slots(ps, own, y, damping, dt, blk, part);
 }
 
-__global__ __launch_bounds__(256) void k_batch_vertex_solve(BatchGatherArgs b) {
-    const int blk = static_cast<int>(blockIdx.x);
-    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, blk);
+__global__ __launch_bounds__(kQ2FusedThreads) void k_q2_fused(Q2FusedArgs a) {
+    const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
+    q2_block(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
+}
+
+__global__ __launch_bounds__(kQ2FusedThreads) void k_batch_q2_fused(BatchQ2FusedArgs b) {
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, bid);
     const SceneDev& sd = b.scenes[seg.scene];
-    if (b.iteration >= sd.iterations) return;
-    const int i = (blk - seg.block_begin) * 256 + threadIdx.x;
-    if (i >= seg.count) return;
-    solve_slot(sd.ps, sd.own, sd.y, sd.partial, sd.damping, b.dt, seg.first + i);
+    if (b.iteration >= sd.iterations) return;     // uniform over the block
+    // the scene's views, read before the first store (which the compiler
+    // could not otherwise tell apart from the scene table)
+    const PSView ps = sd.ps;
+    const FemView fem = sd.fem;
+    const OwnersView own = sd.own;
+    const MaterialView mt = sd.mt;
+    const float4* y = sd.y;
+    const float damping = sd.damping;
+    const int4 blk = gload(sd.fblocks, seg.first + (bid - seg.block_begin));
+    q2_block(ps, fem, own, mt, b.basis, y, damping, b.dt, blk);
 }
 
 }  // namespace
 
-int q2_tasks_per_block() { return kTasksPerBlock; }
-
-void launch_q2_tasks(const Q2TaskArgs& a, hipStream_t s) {
-    if (a.count <= 0) return;
-    const int grid = (a.count + kTasksPerBlock - 1) / kTasksPerBlock;
-    hipLaunchKernelGGL(k_q2_tasks, dim3(grid), dim3(64), 0, s, a);
+void launch_q2_fused(const Q2FusedArgs& a, hipStream_t s) {
+    if (a.count > 0) hipLaunchKernelGGL(k_q2_fused, dim3(a.count), dim3(kQ2FusedThreads), 0, s, a);
 }
 
-void launch_batch_q2_tasks(const BatchQ2TaskArgs& b, int nblocks, hipStream_t s) {
-    if (nblocks > 0) hipLaunchKernelGGL(k_batch_q2_tasks, dim3(nblocks), dim3(64), 0, s, b);
-}
-
-void launch_vertex_solve(const VertexSolveArgs& a, hipStream_t s) {
-    if (a.count <= 0) return;
-    hipLaunchKernelGGL(k_vertex_solve, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
-}
-
-void launch_batch_vertex_solve(const BatchGatherArgs& b, int nblocks, hipStream_t s) {
-    if (nblocks > 0) hipLaunchKernelGGL(k_batch_vertex_solve, dim3(nblocks), dim3(256), 0, s, b);
+void launch_batch_q2_fused(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_batch_q2_fused, dim3(nblocks), dim3(kQ2FusedThreads), 0, s, b);
 }
 
 }  // namespace oe

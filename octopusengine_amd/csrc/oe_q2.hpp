@@ -1,6 +1,6 @@
-// Launch interface of the element-task VBD pass for 27-node hexahedra (see
-// oe_q2.hip): per colour, a task kernel over the colour's incidences writes a
-// 9-float partial per incidence, then a vertex kernel sums them and solves.
+// Launch interface of the fused element-task VBD pass for 27-node hexahedra
+// (see oe_q2.hip): per colour, one kernel whose blocks compute the partials of
+// their vertices' incidences in LDS and then solve those vertices.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,43 +11,40 @@
 
 namespace oe {
 
-struct Q2TaskArgs {
+// One block per vertex block (int4 {first slot, slot count, first owners
+// index, incidence count}, colour-major) with at most kQ2FusedIncPerBlock
+// incidences and kQ2FusedMaxSlots slots.
+constexpr int kQ2FusedThreads = 256;
+constexpr int kQ2FusedIncPerBlock = 32;
+constexpr int kQ2FusedMaxSlots = 64;
+
+struct Q2FusedArgs {
     PSView ps;
     FemView fem;
+    OwnersView own;
     MaterialView mt;
     q2::Basis1D basis;
-    const int* own;    // owners array
-    float* partial;    // 9 floats per owners entry
-    int first;         // first owners index of the colour
-    int count;         // incidences of the colour
-};
-
-struct VertexSolveArgs {
-    PSView ps;
-    OwnersView own;
     const float4* y;
-    const float* partial;
+    const int4* blocks;   // vertex blocks of the scene
     float damping;
     float dt;
-    int first_slot;
+    int first_block;      // the colour's blocks: [first_block, first_block + count)
     int count;
 };
 
-// Batched tasks: BatchSeg.first / count are an owners range of one scene.
-struct BatchQ2TaskArgs {
+// Batched: BatchSeg.first / count are a range of one scene's vertex blocks,
+// one block each.
+struct BatchQ2FusedArgs {
     const SceneDev* scenes;
     const BatchSeg* segs;
     int seg_first;
     int seg_count;
     int iteration;
+    float dt;
     q2::Basis1D basis;
 };
 
-int q2_tasks_per_block();
-void launch_q2_tasks(const Q2TaskArgs& a, hipStream_t s);
-void launch_batch_q2_tasks(const BatchQ2TaskArgs& b, int nblocks, hipStream_t s);
-void launch_vertex_solve(const VertexSolveArgs& a, hipStream_t s);
-// BatchSeg over slots, 256 slots per block
-void launch_batch_vertex_solve(const BatchGatherArgs& b, int nblocks, hipStream_t s);
+void launch_q2_fused(const Q2FusedArgs& a, hipStream_t s);
+void launch_batch_q2_fused(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s);
 
 }  // namespace oe
