@@ -12,7 +12,7 @@ Server::Server(Engine* eng, int max_batch, int nstreams) : eng_(eng), max_batch_
         OE_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
         OE_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     }
-    thread_ = std::thread([this] { worker(); });
+    for (int i = 0; i < nstreams; ++i) threads_.emplace_back([this, i] { worker(i); });
 }
 
 Server::~Server() {
@@ -21,7 +21,8 @@ Server::~Server() {
         stop_ = true;
     }
     cv_.notify_all();
-    if (thread_.joinable()) thread_.join();
+    for (std::thread& t : threads_)
+        if (t.joinable()) t.join();
     for (StreamSlot& s : slots_) {
         if (s.stream) {
             (void)hipStreamSynchronize(s.stream);
@@ -37,7 +38,7 @@ int Server::add_scene(Scene* s) {
     std::unique_lock<std::mutex> lk(mu_);
     if (!error_.empty()) throw std::runtime_error(error_);
     // the device table is reallocated only while nothing is in flight
-    idle_cv_.wait(lk, [&] { return inflight_.empty() || !error_.empty(); });
+    idle_cv_.wait(lk, [&] { return (inflight_.empty() && launching_ == 0) || !error_.empty(); });
     flags_.emplace_back(&eng_->pool, fixed_flags(*s));
     host_table_.push_back(make_scene_dev(*s, flags_.back().get(), tiles_));
     scenes_.push_back(s);
@@ -75,7 +76,7 @@ uint64_t Server::submit(int slot, float dt, int substeps) {
 
 void Server::drain() {
     std::unique_lock<std::mutex> lk(mu_);
-    idle_cv_.wait(lk, [&] { return (pending_.empty() && inflight_.empty()) || !error_.empty(); });
+    idle_cv_.wait(lk, [&] { return (pending_.empty() && inflight_.empty() && launching_ == 0) || !error_.empty(); });
     if (!error_.empty()) throw std::runtime_error(error_);
 }
 
@@ -104,31 +105,32 @@ void Server::retire(bool) {
             vertex_updates_ += t.vertex_updates;
             slots_[t.stream].busy = false;
         }
-        if (pending_.empty() && inflight_.empty()) idle_cv_.notify_all();
+        if (pending_.empty() && inflight_.empty() && launching_ == 0) idle_cv_.notify_all();
     }
 }
 
-bool Server::try_launch(std::unique_lock<std::mutex>& lk) {
-    int si = -1;
-    for (size_t i = 0; i < slots_.size(); ++i)
-        if (!slots_[i].busy) {
-            si = static_cast<int>(i);
-            break;
-        }
-    if (si < 0 || pending_.empty()) return false;
+bool Server::try_launch(std::unique_lock<std::mutex>& lk, int si) {
+    if (slots_[si].busy || pending_.empty()) return false;
     Tick tick;
     tick.stream = si;
     std::vector<char> picked(scenes_.size(), 0);
     float dt = 0.f;
     int sub = 0;
+    int element = -1;
     std::deque<Request> rest;
+    // A tick holds scenes of one element type: every scene of a tick finishes
+    // with the tick's last launch, and a tick's launch count grows with the
+    // colour counts of its types (about 14 for P2 beams, 29 for Q2), so mixing
+    // types would make every request wait for the other type's colours.
     for (Request& r : pending_) {
+        const int el = scenes_[r.slot]->mesh.element;
         const bool fits = static_cast<int>(tick.reqs.size()) < max_batch_ && !scene_busy_[r.slot] && !picked[r.slot] &&
-                          (tick.reqs.empty() || (r.dt == dt && r.substeps == sub));
+                          (tick.reqs.empty() || (r.dt == dt && r.substeps == sub && el == element));
         if (fits) {
             if (tick.reqs.empty()) {
                 dt = r.dt;
                 sub = r.substeps;
+                element = el;
             }
             picked[r.slot] = 1;
             tick.reqs.push_back(r);
@@ -151,6 +153,7 @@ bool Server::try_launch(std::unique_lock<std::mutex>& lk) {
     batch_sum_ += static_cast<int64_t>(tick.reqs.size());
     max_batch_seen_ = std::max<int64_t>(max_batch_seen_, static_cast<int64_t>(tick.reqs.size()));
     const SceneDev* table = table_.get();
+    launching_++;   // holds add_scene() off the table until the tick is in flight
     lk.unlock();
     // plan + launch outside the lock so that submit() is never blocked by it
     slot.sched.build(sc, idx, tiles_);
@@ -161,19 +164,21 @@ bool Server::try_launch(std::unique_lock<std::mutex>& lk) {
     tick.vertex_updates = slot.sched.vertex_updates_per_substep * sub;
     for (Scene* s : sc) s->steps_done++;
     lk.lock();
+    launching_--;
     inflight_.push_back(std::move(tick));
     return true;
 }
 
-void Server::worker() {
+void Server::worker(int si) {
     try {
         OE_HIP(hipSetDevice(eng_->device));
         while (true) {
             retire(false);
             std::unique_lock<std::mutex> lk(mu_);
-            if (stop_) break;
-            if (try_launch(lk)) continue;
-            if (inflight_.empty()) {
+            if (stop_ || !error_.empty()) break;
+            if (try_launch(lk, si)) continue;
+            if (inflight_.empty() && launching_ == 0) {
+                // nothing on the device to retire: sleep until a request comes
                 cv_.wait(lk, [&] { return stop_.load() || !pending_.empty(); });
             } else {
                 lk.unlock();
@@ -182,8 +187,9 @@ void Server::worker() {
         }
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);
-        error_ = e.what();
+        if (error_.empty()) error_ = e.what();
         idle_cv_.notify_all();
+        cv_.notify_all();
     }
 }
 

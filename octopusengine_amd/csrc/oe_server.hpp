@@ -2,16 +2,18 @@
 //
 // There is no counterpart in the reference (it steps every scene once per
 // frame from Engine::update, src/Core/Engine.cpp:15-21).  A request asks for
-// one scene to be advanced by step(dt, substeps).  A worker thread repeatedly
+// one scene to be advanced by step(dt, substeps).  Each stream has a worker
+// thread that repeatedly
 //   * retires finished ticks (HIP event query) and records each request's
 //     latency = completion time - arrival time,
-//   * forms a tick from the oldest pending requests whose scenes are not
-//     already on the device (one request per scene per tick, up to max_batch
-//     scenes, same dt/substeps),
-//   * launches the tick on a free stream as one batched step (the launch plan
-//     of oe_batch) - so requests that arrive while others run are admitted as
-//     soon as a stream slot and their scene are free, and ticks on different
-//     streams overlap on the device.
+//   * when its stream is free, forms a tick from the oldest pending requests
+//     whose scenes are not already on the device (one request per scene per
+//     tick, up to max_batch scenes, same dt/substeps, one element type),
+//   * launches the tick on its stream as one batched step (the launch plan of
+//     oe_batch) - so requests that arrive while others run are admitted as
+//     soon as a stream and their scene are free, ticks on different streams
+//     overlap on the device, and the host-side launch work of different
+//     streams runs in parallel.
 // Scene state never leaves HBM; scenes are admitted against the engine's
 // memory budget when they are created.
 #pragma once
@@ -70,8 +72,8 @@ private:
         bool busy = false;
     };
 
-    void worker();
-    bool try_launch(std::unique_lock<std::mutex>& lk);
+    void worker(int si);
+    bool try_launch(std::unique_lock<std::mutex>& lk, int si);
     void retire(bool block);
     void grow_table();
 
@@ -87,6 +89,7 @@ private:
     std::vector<char> scene_busy_;
     std::deque<Request> pending_;
     std::vector<Tick> inflight_;
+    int launching_ = 0;   // ticks taken from pending_ and not yet in inflight_
     std::mutex mu_;
     std::condition_variable cv_;
     std::condition_variable idle_cv_;
@@ -95,7 +98,7 @@ private:
     int64_t submitted_ = 0, completed_ = 0, vertex_updates_ = 0, batches_ = 0, batch_sum_ = 0, max_batch_seen_ = 0;
     std::vector<double> lat_ms_;
     std::string error_;
-    std::thread thread_;
+    std::vector<std::thread> threads_;
 };
 
 }  // namespace oe
