@@ -250,9 +250,57 @@ oe_status oe_vtk_cell_data(oe_vtk* v);
 oe_status oe_vtk_scalars(oe_vtk* v, const char* name, const float* data, int32_t n);
 oe_status oe_vtk_vectors(oe_vtk* v, const char* name, const float* data, int32_t n);
 oe_status oe_vtk_close(oe_vtk* v);
-/* Scene snapshot as the reference's result files hold it: the rest geometry,
- * point data "weights" (vertex masses) and "u" (displacement). Synchronising. */
+/* Scene snapshot in the layout of FEM_VTK_Recorder::save
+ * (src/Script/Record/Recorder.cpp:117-171): the rest geometry; point data
+ * "weights" (vertex masses), "u_norm" (|u|), "u" (displacement),
+ * "smooth_stress" (sum of the incident elements' stress / vertex mass); cell
+ * data "stress", "volume", "volume_diff", "inverted" (the element diagnostics
+ * above).  The reference's recorder copies its "inverted" array onto itself
+ * (:137-141) and so writes zeros; this writes the flags.  Synchronising. */
 oe_status oe_scene_save_vtk(oe_scene* s, const char* path);
+
+/* ------------------------------------------------------------ JSON record */
+/* The reference's DataRecorder file (src/Script/Record/DataRecorder.cpp:16-38):
+ * {"name" : value,...} in the order the entries are added, each value written
+ * as its recorder writes it (std::ostream default formatting):
+ *   time      TimeRecorder      (include/Script/Record/Recorder.h:37-47)
+ *   mesh      MeshRecorder      (src/Script/Record/Recorder.cpp:13-24); counts[element id], 10 entries
+ *   material  FEM_Dynamic_Recorder (xpbd = 0: sub_step) or XPBD_FEM_Dynamic_Recorder
+ *             (xpbd = 1: iteration, sub_itetrations) (:27-49), both named "XPBD_FEM_Dynamic"
+ *   string    the VTK recorders' file entries (:95-97, 113-115, 173-175)
+ *   flexion   FEM_Flexion_error_recorder (:210-219)
+ *   rotation  FEM_Torsion_error_recorder (:317-341)
+ * close writes the file and frees the handle; text copies the JSON text into
+ * buf (capacity cap bytes, NUL-terminated) and sets *len to its length. */
+typedef struct oe_record oe_record;
+oe_status oe_record_open(const char* path, oe_record** out);
+oe_status oe_record_time(oe_record* r, int32_t frame, float time, float dt, float fixed_time, float fixed_dt);
+oe_status oe_record_mesh(oe_record* r, const int32_t* counts, int32_t n_vertices);
+oe_status oe_record_material(oe_record* r, int32_t xpbd, int32_t material, float poisson, float young,
+                             int32_t iterations, int32_t sub_iterations, float density);
+oe_status oe_record_string(oe_record* r, const char* name, const char* value);
+oe_status oe_record_flexion(oe_record* r, float error, const float* p, const float* target);
+oe_status oe_record_rotation(oe_record* r, float rot_max, float beam_length, const float* dist, const float* angles,
+                             int32_t n);
+oe_status oe_record_text(oe_record* r, char* buf, int64_t cap, int64_t* len);
+oe_status oe_record_close(oe_record* r);
+
+/* Error measures of the reference's recorders, on host arrays (3 floats per
+ * vertex).  Flexion (FEM_Flexion_error_recorder::init, Recorder.cpp:190-208):
+ * the first vertex whose rest position is within squared distance 1e-6 of
+ * follow; *error = |target - p|^2, p its current position, *vertex its index
+ * (-1 and no outputs when there is none).  Rotation
+ * (FEM_Torsion_error_recorder::init + compute_errors, Recorder.cpp:221-315):
+ * samples_per_edge samples on every element edge lying on y = z = 0 in the rest
+ * state; dist = current x / beam_length, angle (degrees) between the rest and
+ * current (y, z) directions from the axis (0.5, 0.5); sorted by dist, runs
+ * closer than 1e-4 merged.  *n: capacity in, count out (OE_ERR_INVALID when
+ * the capacity is short). */
+oe_status oe_flexion_error(const float* rest, const float* pos, int32_t n_vertices, const float* follow,
+                           const float* target, float* error, float* p, int32_t* vertex);
+oe_status oe_rotation_error(int32_t element, const int32_t* topology, int32_t n_elements, const float* rest,
+                            const float* pos, int32_t n_vertices, int32_t samples_per_edge, float beam_length,
+                            float* dist, float* angles, int32_t* n);
 
 #ifdef __cplusplus
 }

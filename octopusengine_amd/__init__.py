@@ -35,6 +35,8 @@ __all__ = [
     "MASS_UNIFORM", "MASS_FEMSHAPE", "SOLVER_VBD", "SOLVER_XPBD", "SOLVER_EXPLICIT",
     "VBD_BASE", "VBD_THREADED_QUADRATURE", "VBD_REDUCTION_SYMMETRY", "VBD_BETTER_COLORATION",
     "XPBD_XPBD", "XPBD_QUADRATURE", "XPBD_OPTISHARED", "XPBD_OPTIGROUP", "NODES",
+    "VtkWriter", "Recorder", "load_mesh", "flexion_error", "rotation_error",
+    "PASS_GATHER", "PASS_CACHE", "PASS_TASKS",
 ]
 
 
@@ -173,6 +175,87 @@ class VtkWriter:
         if getattr(self, "_h", None):
             h, self._h = self._h, None
             L.check(L.lib.oe_vtk_close(h))
+
+
+class Recorder:
+    """The reference's DataRecorder JSON file ({"name" : value,...}, entries in
+    the order added, each written as its recorder writes it).  `save()` writes
+    the file and frees the handle; `text()` returns the JSON text."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        L.check(L.lib.oe_record_open(str(path).encode(), C.byref(h)))
+        self._h = h
+
+    def time(self, frame: int, time: float, dt: float, fixed_time: float, fixed_dt: float):
+        L.check(L.lib.oe_record_time(self._h, frame, time, dt, fixed_time, fixed_dt))
+        return self
+
+    def mesh(self, counts: dict, n_vertices: int):
+        """counts: {element id: number of elements}"""
+        c = np.zeros(10, np.int32)
+        for e, n in counts.items():
+            c[int(e)] = int(n)
+        L.check(L.lib.oe_record_mesh(self._h, L.iptr(c), int(n_vertices)))
+        return self
+
+    def material(self, material: int, poisson: float, young: float, sub_iterations: int, density: float,
+                 iterations: int | None = None):
+        """FEM_Dynamic_Recorder, or XPBD_FEM_Dynamic_Recorder when iterations is given."""
+        xpbd = iterations is not None
+        L.check(L.lib.oe_record_material(self._h, int(xpbd), material, poisson, young,
+                                         int(iterations) if xpbd else 0, sub_iterations, density))
+        return self
+
+    def string(self, name: str, value: str):
+        L.check(L.lib.oe_record_string(self._h, name.encode(), value.encode()))
+        return self
+
+    def flexion(self, error: float, p, target):
+        p, target = _f32(p).reshape(3), _f32(target).reshape(3)
+        L.check(L.lib.oe_record_flexion(self._h, error, L.fptr(p), L.fptr(target)))
+        return self
+
+    def rotation(self, rot_max: float, beam_length: float, dist, angles):
+        d, a = _f32(dist).reshape(-1), _f32(angles).reshape(-1)
+        if d.size != a.size:
+            raise ValueError("dist and angles differ in length")
+        L.check(L.lib.oe_record_rotation(self._h, rot_max, beam_length, L.fptr(d), L.fptr(a), d.size))
+        return self
+
+    def text(self) -> str:
+        n = C.c_int64()
+        L.check(L.lib.oe_record_text(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        L.check(L.lib.oe_record_text(self._h, buf, n.value + 1, C.byref(n)))
+        return buf.raw[:n.value].decode()
+
+    def save(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            L.check(L.lib.oe_record_close(h))
+
+
+def flexion_error(rest, pos, follow, target):
+    """(error, p, vertex) of FEM_Flexion_error_recorder; vertex -1 when no rest
+    position is within 1e-3 of follow."""
+    rest, pos = _f32(rest).reshape(-1, 3), _f32(pos).reshape(-1, 3)
+    f, t = _f32(follow).reshape(3), _f32(target).reshape(3)
+    err, p, v = C.c_float(), np.zeros(3, np.float32), C.c_int32()
+    L.check(L.lib.oe_flexion_error(L.fptr(rest), L.fptr(pos), rest.shape[0], L.fptr(f), L.fptr(t), C.byref(err),
+                                   L.fptr(p), C.byref(v)))
+    return float(err.value), p, int(v.value)
+
+
+def rotation_error(element: int, topo, rest, pos, samples_per_edge: int, beam_length: float):
+    """(dist, angles) of FEM_Torsion_error_recorder.compute_errors."""
+    topo = _i32(topo)
+    rest, pos = _f32(rest).reshape(-1, 3), _f32(pos).reshape(-1, 3)
+    cap = max(1, topo.shape[0] * max(2, int(samples_per_edge)) * 4)
+    d, a, n = np.zeros(cap, np.float32), np.zeros(cap, np.float32), C.c_int32(cap)
+    L.check(L.lib.oe_rotation_error(element, L.iptr(topo), topo.shape[0], L.fptr(rest), L.fptr(pos), rest.shape[0],
+                                    int(samples_per_edge), beam_length, L.fptr(d), L.fptr(a), C.byref(n)))
+    return d[:n.value].copy(), a[:n.value].copy()
 
 
 # VBD colour-pass modes (oe_scene_set_tile)

@@ -136,6 +136,19 @@ _SIGS = {
     "oe_vtk_vectors": (C.c_int32, [P, C.c_char_p, f32p, C.c_int32]),
     "oe_vtk_close": (C.c_int32, [P]),
     "oe_scene_save_vtk": (C.c_int32, [P, C.c_char_p]),
+    "oe_record_open": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "oe_record_time": (C.c_int32, [P, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "oe_record_mesh": (C.c_int32, [P, i32p, C.c_int32]),
+    "oe_record_material": (C.c_int32, [P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, C.c_int32,
+                                       C.c_float]),
+    "oe_record_string": (C.c_int32, [P, C.c_char_p, C.c_char_p]),
+    "oe_record_flexion": (C.c_int32, [P, C.c_float, f32p, f32p]),
+    "oe_record_rotation": (C.c_int32, [P, C.c_float, C.c_float, f32p, f32p, C.c_int32]),
+    "oe_record_text": (C.c_int32, [P, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "oe_record_close": (C.c_int32, [P]),
+    "oe_flexion_error": (C.c_int32, [f32p, f32p, C.c_int32, f32p, f32p, f32p, f32p, C.POINTER(C.c_int32)]),
+    "oe_rotation_error": (C.c_int32, [C.c_int32, i32p, C.c_int32, f32p, f32p, C.c_int32, C.c_int32, C.c_float,
+                                      f32p, f32p, C.POINTER(C.c_int32)]),
     "oe_host_affine_records": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, f32p, f32p, i32p]),
 }
 

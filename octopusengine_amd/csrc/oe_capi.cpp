@@ -3,6 +3,7 @@
 #include "../../include/octopus_engine.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -25,6 +26,10 @@ struct oe_batch {
 struct oe_server {
     std::unique_ptr<oe::Server> server;
     oe_engine* owner;
+};
+struct oe_record {
+    std::string path;
+    oe::JsonRecord rec;
 };
 struct oe_vtk {
     std::unique_ptr<oe::VtkWriter> w;
@@ -742,12 +747,133 @@ oe_status oe_scene_save_vtk(oe_scene* s, const char* path) {
         std::vector<float> u(pos.size());
         for (size_t i = 0; i < u.size(); ++i) u[i] = pos[i] - rest[i];
         const int nv = sc.mesh.nb_vertices();
+        const int ne = sc.mesh.nb_elements();
+        const int S = oe::elem_nb_vertices(sc.mesh.element);
+        std::vector<float> u_norm(nv), stress(ne), volume(ne), volume_diff(ne), inverted(ne);
+        for (int i = 0; i < nv; ++i)
+            u_norm[i] = std::sqrt(u[3 * i] * u[3 * i] + u[3 * i + 1] * u[3 * i + 1] + u[3 * i + 2] * u[3 * i + 2]);
+        sc.fem->diagnostics(sc.ps.get(), stress.data(), volume.data(), volume_diff.data(), inverted.data());
+        // Cuda_FEM_Dynamic::get_stress_vertices (include/Script/Dynamic/Cuda_FEM_Dynamic.h:96-117)
+        std::vector<float> smooth(nv, 0.f);
+        for (int e = 0; e < ne; ++e)
+            for (int j = 0; j < S; ++j) smooth[sc.mesh.topo[static_cast<size_t>(e) * S + j]] += stress[e];
+        for (int i = 0; i < nv; ++i) smooth[i] /= m[i];
         oe::VtkWriter w(path);
-        w.mesh(sc.mesh.element, rest.data(), nv, sc.mesh.topo.data(), sc.mesh.nb_elements());
+        w.mesh(sc.mesh.element, rest.data(), nv, sc.mesh.topo.data(), ne);
         w.point_data();
         w.scalars("weights", m.data(), nv);
+        w.scalars("u_norm", u_norm.data(), nv);
         w.vectors("u", u.data(), nv);
+        w.scalars("smooth_stress", smooth.data(), nv);
+        w.cell_data();
+        w.scalars("stress", stress.data(), ne);
+        w.scalars("volume", volume.data(), ne);
+        w.scalars("volume_diff", volume_diff.data(), ne);
+        w.scalars("inverted", inverted.data(), ne);
         w.close();
+    });
+}
+
+// ------------------------------------------------------------------ JSON record
+oe_status oe_record_open(const char* path, oe_record** out) {
+    return guard([&] {
+        OE_REQUIRE(path && out, "bad argument");
+        auto r = std::make_unique<oe_record>();
+        r->path = path;
+        *out = r.release();
+    });
+}
+
+oe_status oe_record_time(oe_record* r, int32_t frame, float time, float dt, float fixed_time, float fixed_dt) {
+    return guard([&] {
+        OE_REQUIRE(r, "bad argument");
+        r->rec.time(frame, time, dt, fixed_time, fixed_dt);
+    });
+}
+
+oe_status oe_record_mesh(oe_record* r, const int32_t* counts, int32_t n_vertices) {
+    return guard([&] {
+        OE_REQUIRE(r && counts && n_vertices >= 0, "bad argument");
+        r->rec.mesh(counts, n_vertices);
+    });
+}
+
+oe_status oe_record_material(oe_record* r, int32_t xpbd, int32_t material, float poisson, float young,
+                             int32_t iterations, int32_t sub_iterations, float density) {
+    return guard([&] {
+        OE_REQUIRE(r, "bad argument");
+        OE_REQUIRE(material >= 0 && material <= 3, "unknown material");
+        r->rec.material(xpbd != 0, material, poisson, young, iterations, sub_iterations, density);
+    });
+}
+
+oe_status oe_record_string(oe_record* r, const char* name, const char* value) {
+    return guard([&] {
+        OE_REQUIRE(r && name && value, "bad argument");
+        r->rec.string(name, value);
+    });
+}
+
+oe_status oe_record_flexion(oe_record* r, float error, const float* p, const float* target) {
+    return guard([&] {
+        OE_REQUIRE(r && p && target, "bad argument");
+        r->rec.flexion(error, p, target);
+    });
+}
+
+oe_status oe_record_rotation(oe_record* r, float rot_max, float beam_length, const float* dist, const float* angles,
+                             int32_t n) {
+    return guard([&] {
+        OE_REQUIRE(r && n >= 0 && (n == 0 || (dist && angles)), "bad argument");
+        r->rec.rotation(rot_max, beam_length, dist, angles, n);
+    });
+}
+
+oe_status oe_record_text(oe_record* r, char* buf, int64_t cap, int64_t* len) {
+    return guard([&] {
+        OE_REQUIRE(r && len, "bad argument");
+        const std::string t = r->rec.text();
+        *len = static_cast<int64_t>(t.size());
+        if (buf) {
+            OE_REQUIRE(cap > static_cast<int64_t>(t.size()), "buffer too small");
+            std::memcpy(buf, t.c_str(), t.size() + 1);
+        }
+    });
+}
+
+oe_status oe_record_close(oe_record* r) {
+    return guard([&] {
+        OE_REQUIRE(r, "bad argument");
+        std::unique_ptr<oe_record> own(r);
+        own->rec.save(own->path);
+    });
+}
+
+oe_status oe_flexion_error(const float* rest, const float* pos, int32_t n_vertices, const float* follow,
+                           const float* target, float* error, float* p, int32_t* vertex) {
+    return guard([&] {
+        OE_REQUIRE(rest && pos && follow && target && error && p && vertex && n_vertices >= 0, "bad argument");
+        *vertex = oe::flexion_error(rest, pos, n_vertices, follow, target, *error, p);
+    });
+}
+
+oe_status oe_rotation_error(int32_t element, const int32_t* topology, int32_t n_elements, const float* rest,
+                            const float* pos, int32_t n_vertices, int32_t samples_per_edge, float beam_length,
+                            float* dist, float* angles, int32_t* n) {
+    return guard([&] {
+        OE_REQUIRE(topology && rest && pos && dist && angles && n && n_elements >= 0, "bad argument");
+        const oe::Element el = static_cast<oe::Element>(element);
+        OE_REQUIRE(el == oe::Tetra || el == oe::Hexa || el == oe::Tetra10 || el == oe::Tetra20 || el == oe::Hexa27,
+                   "unsupported element");
+        const int S = oe::elem_nb_vertices(el);
+        for (int64_t i = 0; i < static_cast<int64_t>(n_elements) * S; ++i)
+            OE_REQUIRE(topology[i] >= 0 && topology[i] < n_vertices, "topology index out of range");
+        std::vector<float> d, a;
+        oe::rotation_error(el, topology, n_elements, rest, pos, samples_per_edge, beam_length, d, a);
+        OE_REQUIRE(static_cast<int64_t>(d.size()) <= *n, "output capacity too small");
+        std::copy(d.begin(), d.end(), dist);
+        std::copy(a.begin(), a.end(), angles);
+        *n = static_cast<int32_t>(d.size());
     });
 }
 

@@ -196,4 +196,29 @@ private:
     int mode_ = 0;   // 1 point data, 2 cell data
 };
 
+// ------------------------------------------------------------ JSON record
+// The reference's DataRecorder file: {"name" : value,...} in insertion order,
+// each value as its recorder writes it (oe_record.cpp).
+class JsonRecord {
+public:
+    void time(int frame, float time, float dt, float fixed_time, float fixed_dt);
+    void mesh(const int32_t counts[10], int n_vertices);
+    void material(bool xpbd, int material, float poisson, float young, int iterations, int sub_iterations,
+                  float density);
+    void string(const std::string& name, const std::string& value);
+    void flexion(float error, const float p[3], const float target[3]);
+    void rotation(float rot_max, float beam_length, const float* dist, const float* angles, int n);
+    std::string text() const;
+    void save(const std::string& path) const;
+
+private:
+    void entry(const std::string& name, const std::string& value);
+    std::vector<std::pair<std::string, std::string>> entries_;
+};
+
+int flexion_error(const float* rest, const float* pos, int n, const float follow[3], const float target[3],
+                  float& error, float p[3]);
+void rotation_error(Element el, const int32_t* topo, int ne, const float* rest, const float* pos, int samples,
+                    float beam_length, std::vector<float>& dist, std::vector<float>& angles);
+
 }  // namespace oe

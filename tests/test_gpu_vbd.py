@@ -287,3 +287,65 @@ def test_vbd_free_fall_closed_form(engine):
     expect = rest.astype(np.float64).copy()
     expect[:, 1] += -9.81 * dt * dt
     assert max_err(p, expect) <= 1e-6
+
+
+def _vtk_data(text):
+    """{name: array} of the SCALARS / VECTORS blocks of a legacy-VTK file, and
+    the block names in order."""
+    toks = text.split("\n")
+    out, order = {}, []
+    n_point = n_cell = 0
+    mode = None
+    i = 0
+    while i < len(toks):
+        line = toks[i]
+        if line.startswith("POINT_DATA"):
+            n_point, mode = int(line.split()[1]), "p"
+        elif line.startswith("CELL_DATA"):
+            n_cell, mode = int(line.split()[1]), "c"
+        elif line.startswith("SCALARS"):
+            name = line.split()[1]
+            out[name] = np.array(toks[i + 2].split(), np.float32)
+            assert out[name].size == (n_point if mode == "p" else n_cell)
+            order.append(name)
+            i += 2
+        elif line.startswith("VECTORS"):
+            name = line.split()[1]
+            out[name] = np.array(toks[i + 1].split(), np.float32).reshape(-1, 3)
+            assert out[name].shape[0] == n_point
+            order.append(name)
+            i += 1
+        i += 1
+    return out, order
+
+
+def test_save_vtk_fem_record_fields(engine, tmp_path):
+    """save_vtk writes FEM_VTK_Recorder's fields: weights, u_norm, u,
+    smooth_stress (point data); stress, volume, volume_diff, inverted (cell
+    data).  Values vs the engine's own getters and the oracle's diagnostics."""
+    import octopusengine_amd as oe
+
+    g, o, _, _ = make_pair(engine, oe.TETRA10, (4, 2, 2), iterations=3, young=1e5, poisson=0.45)
+    for _ in range(3):
+        g.step(0.005, 1)
+        o.step(0.005, 1)
+    path = tmp_path / "s.vtk"
+    g.save_vtk(path)
+    d, order = _vtk_data(path.read_text())
+    assert order == ["weights", "u_norm", "u", "smooth_stress", "stress", "volume", "volume_diff", "inverted"]
+    rest, pos, m = g.init_positions(), g.positions(), g.masses()
+    u = pos - rest
+    assert np.allclose(d["weights"], m, rtol=1e-5)
+    assert np.allclose(d["u"], u, rtol=1e-5, atol=1e-9)
+    assert np.allclose(d["u_norm"], np.linalg.norm(u, axis=1), rtol=1e-5, atol=1e-9)
+    do = o.element_diagnostics()
+    for k in ("stress", "volume", "volume_diff"):
+        scale = float(np.abs(do[k]).max())
+        tol = 1e-4 * scale if k == "stress" else 1e-5 * float(np.abs(do["volume"]).max())
+        assert float(np.abs(d[k] - do[k]).max()) <= tol, k
+    assert np.array_equal(d["inverted"], do["inverted"].astype(np.float32))
+    # smooth_stress: sum of the incident elements' stress over the vertex mass
+    topo = oe.beam_mesh(oe.TETRA10, (4, 2, 2))[1]
+    acc = np.zeros(len(rest), np.float64)
+    np.add.at(acc, topo.reshape(-1), np.repeat(d["stress"].astype(np.float64), topo.shape[1]))
+    assert np.allclose(d["smooth_stress"], acc / m, rtol=1e-4, atol=1e-6 * float(np.abs(acc / m).max()))
