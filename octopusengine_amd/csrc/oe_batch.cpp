@@ -62,10 +62,11 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
     if (t.mode == PASS_TASKS && s.mesh.element == Hexa27 && !s.vbd_fem->has_q2_basis)
         throw std::invalid_argument("no tensor factors for this 27-node hexahedron");
     d.ju = t.mode == PASS_CACHE ? s.vbd_fem->ensure_ju() : nullptr;
-    d.fblocks = nullptr;
+    d.fblocks = d.fblocks_lat = nullptr;
     if (t.mode == PASS_TASKS) {
         s.vbd_fem->prepare_tasks();
         d.fblocks = s.vbd_fem->d_fblk.get();
+        d.fblocks_lat = s.vbd_fem->d_fblk_lat.get();
     }
     d.ps = s.ps->get_parameters();
     d.fem = s.fem->fem_view();
@@ -106,7 +107,7 @@ void Schedule::clear() {
 }
 
 void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& table_index,
-                     const std::map<int, GatherTile>& tiles) {
+                     const std::map<int, GatherTile>& tiles, bool want_latency) {
     clear();
     int max_colors = 0;
     for (Scene* s : scenes) {
@@ -123,6 +124,20 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
             GLaunch gl{};
             gl.color = c;
             gl.seg_first = static_cast<int>(segs.size());
+            // Task passes take the latency form when it was asked for, every
+            // scene of the launch has its vertex blocks, and the throughput
+            // form would give fewer than 2 blocks per CU.
+            if (want_latency && tile_of(tiles, el).mode == PASS_TASKS) {
+                bool all = true;
+                int64_t through_blocks = 0;
+                for (Scene* s : scenes) {
+                    if (s->mesh.element != el || c >= s->vbd_fem->nb_colors()) continue;
+                    if (!s->vbd_fem->d_fblk_lat.get()) all = false;
+                    const std::vector<int32_t>& fb = s->vbd_fem->fblk_color_begin;
+                    if (!fb.empty()) through_blocks += fb[c + 1] - fb[c];
+                }
+                gl.latency = all && through_blocks < kLatencyBlocks;
+            }
             bool any = false;
             for (size_t i = 0; i < scenes.size(); ++i) {
                 Scene* s = scenes[i];
@@ -142,7 +157,8 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                 }
                 if (gl.mode == PASS_TASKS) {
                     // fused tasks: one block per vertex block of the colour
-                    const std::vector<int32_t>& fb = s->vbd_fem->fblk_color_begin;
+                    const std::vector<int32_t>& fb =
+                        gl.latency ? s->vbd_fem->fblk_lat_color_begin : s->vbd_fem->fblk_color_begin;
                     if (fb.empty()) throw std::logic_error("vertex blocks not prepared");
                     const int nfb = fb[c + 1] - fb[c];
                     segs.push_back(BatchSeg{table_index[i], fb[c], nfb, gl.nblocks});
@@ -244,8 +260,8 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
     BatchVArgs v{d_scenes, dev.vsegs.get(), static_cast<int>(sch.vsegs.size()), h, 0};
     BatchGatherArgs g{d_scenes, dev.segs.get(), 0, 0, 0, h};
     BatchRefreshArgs r{d_scenes, dev.jsegs.get(), 0, 0, 0};
-    BatchQ2FusedArgs tk{d_scenes, dev.segs.get(), 0, 0, 0, h, sch.q2_basis};
-    BatchFusedArgs ft{d_scenes, dev.segs.get(), 0, 0, 0, h};
+    BatchQ2FusedArgs tk{d_scenes, dev.segs.get(), 0, 0, 0, h, sch.q2_basis, 0};
+    BatchFusedArgs ft{d_scenes, dev.segs.get(), 0, 0, 0, h, 0};
     for (int sub = 0; sub < substeps; ++sub) {
         launch_batch_predict(v, sch.vblocks, stream);
         for (int it = 0; it < sch.max_iterations; ++it) {
@@ -263,10 +279,12 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                 if (gl.mode == PASS_TASKS && gl.S == 27 && gl.Q == 27) {
                     tk.seg_first = gl.seg_first;
                     tk.seg_count = gl.seg_count;
+                    tk.latency = gl.latency ? 1 : 0;
                     launch_batch_q2_fused(tk, gl.nblocks, stream);
                 } else if (gl.mode == PASS_TASKS) {
                     ft.seg_first = gl.seg_first;
                     ft.seg_count = gl.seg_count;
+                    ft.latency = gl.latency ? 1 : 0;
                     if (!launch_batch_fused_tasks(gl.S, gl.Q, ft, gl.nblocks, stream))
                         throw std::invalid_argument("unsupported element in batch");
                 } else if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.mode == PASS_CACHE, g, gl.nblocks,

@@ -27,6 +27,10 @@
 
 namespace oe {
 
+// Below this many throughput-form blocks (2 per CU) a task pass launch cannot
+// fill the GPU, and a schedule built for latency uses the latency form.
+constexpr int64_t kLatencyBlocks = 512;
+
 // Launch plan for a set of scenes.  Scene indices refer to a SceneDev table.
 // Every launch covers a contiguous run of segments whose block_begin values
 // are launch-local; kernels find their segment by binary search, so the plan
@@ -35,6 +39,7 @@ struct Schedule {
     struct GLaunch {
         int color, S, Q, L, QL;
         int mode;                         // PassMode
+        bool latency;                     // task pass in the kernels' latency form
         int seg_first, seg_count, nblocks;   // gather: slot ranges; tasks: vertex-block ranges
         int64_t slots;   // vertex slots covered (all scenes)
         int64_t incidences;
@@ -56,9 +61,11 @@ struct Schedule {
     double algorithmic_bytes_per_substep = 0.0;   // gather kernels, all iterations
 
     void clear();
-    // scenes[i] is placed at SceneDev table index table_index[i]
+    // scenes[i] is placed at SceneDev table index table_index[i]; want_latency
+    // lets task passes too small to fill the GPU (as a server's ticks often
+    // are) take the latency form
     void build(const std::vector<Scene*>& scenes, const std::vector<int>& table_index,
-               const std::map<int, GatherTile>& tiles);
+               const std::map<int, GatherTile>& tiles, bool want_latency = false);
 };
 
 // Device copies of a schedule's tables (grown on demand, reused across uploads).

@@ -211,6 +211,33 @@ struct OwnersView {
     const int* own;         // (eid << 5) | local node
 };
 
+// What the vertex solve of one slot reads.  The latency forms of the fused
+// task kernels load it before the element work of their block, so that the
+// loads overlap that work (a slot's vertex is written only by the block that
+// solves it).
+struct SlotState {
+    int vid;
+    int n_inc, first_inc;   // incident elements, first owners index
+    int free;               // mask (0: fixed, not solved)
+    float4 p, last_p, y;
+    float m;
+};
+
+__device__ __forceinline__ SlotState load_slot_state(const PSView& ps, const OwnersView& own, const float4* y,
+                                                     int slot) {
+    SlotState st;
+    st.vid = gload(own.slot_vid, slot);
+    const int2 nbo = gload(own.slot_nbo, slot);
+    st.n_inc = nbo.x;
+    st.first_inc = nbo.y;
+    st.free = gload(ps.mask, st.vid);
+    st.p = gload(static_cast<const float4*>(ps.p), st.vid);
+    st.last_p = gload(static_cast<const float4*>(ps.last_p), st.vid);
+    st.y = gload(y, st.vid);
+    st.m = gload(ps.m, st.vid);
+    return st;
+}
+
 struct MaterialView {
     int material;
     float lambda, mu, alpha;   // alpha = 1 (NeoHooke) or 1 + mu/lambda (Stable NeoHooke)

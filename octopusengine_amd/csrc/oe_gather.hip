@@ -376,81 +376,97 @@ __device__ __forceinline__ void elem_partial(const FemView& f, const MaterialVie
     }
 }
 
+// Phase 1 of a fused block: the partials of the block's incidences into LDS.
 template <int S, int Q, class Grad>
+__device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f, const OwnersView& own,
+                                             const MaterialView& mt, const Grad& grad, const int4 blk,
+                                             float (*part)[9]) {
+    const int first_own = blk.z, n_inc = blk.w;
+    const int t0 = threadIdx.x;
+    if (t0 >= n_inc) return;
+    const int last = first_own + n_inc - 1;
+    int pk = gload(own.own, first_own + t0);
+    int tp[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
+    int pk_next = gload(own.own, min(first_own + t0 + kFusedThreads, last));
+#pragma unroll 1
+    for (int t = t0; t < n_inc; t += kFusedThreads) {
+        float xs[S][3];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const float4 x = gload(static_cast<const float4*>(ps.p), tp[k]);
+            xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
+        }
+        const int eid = pk >> 5, r = pk & 31;
+        pk = pk_next;
+#pragma unroll
+        for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
+        pk_next = gload(own.own, min(first_own + t + 2 * kFusedThreads, last));
+        Acc9 acc;
+        elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) part[t][i] = acc.v[i];
+    }
+}
+
+template <int S, int Q, bool LAT, class Grad>
 __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, const OwnersView& own,
                                             const MaterialView& mt, const Grad& grad, const float4* y, float damping,
                                             float dt, const int4 blk, float (*part)[9]) {
-    const int first_slot = blk.x, n_slots = blk.y, first_own = blk.z, n_inc = blk.w;
-    // phase 1: partials of the block's incidences into LDS
+    const int first_slot = blk.x, n_slots = blk.y, first_own = blk.z;
     const int t0 = threadIdx.x;
-    if (t0 < n_inc) {
-        const int last = first_own + n_inc - 1;
-        int pk = gload(own.own, first_own + t0);
-        int tp[S];
+    if constexpr (LAT) {
+        SlotState st{};
+        if (t0 < n_slots) st = load_slot_state(ps, own, y, first_slot + t0);
+        fused_phase1<S, Q>(ps, f, own, mt, grad, blk, part);
+        __syncthreads();
+        if (t0 < n_slots && st.free != 0) solve_slot_from_lds(ps, st, part, first_own, damping, dt);
+    } else {
+        fused_phase1<S, Q>(ps, f, own, mt, grad, blk, part);
+        __syncthreads();
+        // phase 2: one thread per slot, partials in the owners order, local solve
+        if (t0 >= n_slots) return;
+        const int slot = first_slot + t0;
+        const int vid = gload(own.slot_vid, slot);
+        if (gload(ps.mask, vid) == 0) return;
+        const int2 nbo = gload(own.slot_nbo, slot);
+        Acc9 acc;
+        acc_zero(acc);
+        for (int j = nbo.y - first_own, e = j + nbo.x; j < e; ++j) {
 #pragma unroll
-        for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
-        int pk_next = gload(own.own, min(first_own + t0 + kFusedThreads, last));
-#pragma unroll 1
-        for (int t = t0; t < n_inc; t += kFusedThreads) {
-            float xs[S][3];
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const float4 x = gload(static_cast<const float4*>(ps.p), tp[k]);
-                xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
-            }
-            const int eid = pk >> 5, r = pk & 31;
-            pk = pk_next;
-#pragma unroll
-            for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
-            pk_next = gload(own.own, min(first_own + t + 2 * kFusedThreads, last));
-            Acc9 acc;
-            elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) part[t][i] = acc.v[i];
+            for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
         }
+        const float4 pv = gload(static_cast<const float4*>(ps.p), vid);
+        double dx[3];
+        vbd_local_solve(acc, pv, gload(static_cast<const float4*>(ps.last_p), vid), gload(y, vid), gload(ps.m, vid),
+                        damping, dt, dx);
+        gstore(ps.p, vid,
+               make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
+                           pv.z + static_cast<float>(dx[2]), pv.w));
     }
-    __syncthreads();
-    // phase 2: one thread per slot, partials in the owners order, local solve
-    if (t0 >= n_slots) return;
-    const int slot = first_slot + t0;
-    const int vid = gload(own.slot_vid, slot);
-    if (gload(ps.mask, vid) == 0) return;
-    const int2 nbo = gload(own.slot_nbo, slot);
-    Acc9 acc;
-    acc_zero(acc);
-    for (int j = nbo.y - first_own, e = j + nbo.x; j < e; ++j) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
-    }
-    const float4 pv = gload(static_cast<const float4*>(ps.p), vid);
-    double dx[3];
-    vbd_local_solve(acc, pv, gload(static_cast<const float4*>(ps.last_p), vid), gload(y, vid), gload(ps.m, vid),
-                    damping, dt, dx);
-    gstore(ps.p, vid,
-           make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
-                       pv.z + static_cast<float>(dx[2]), pv.w));
 }
 
-template <int S, int Q>
+template <int S, int Q, bool LAT>
 __device__ __forceinline__ void fused_dispatch(const PSView& ps, const FemView& f, const OwnersView& own,
                                                const MaterialView& mt, const ShapeGradArg<S, Q>& g, const float4* y,
                                                float damping, float dt, const int4 blk, float (*part)[9]) {
     if constexpr (grad_by_value<S, Q>())
-        fused_block<S, Q>(ps, f, own, mt, GradByValue<S, Q>{g}, y, damping, dt, blk, part);
+        fused_block<S, Q, LAT>(ps, f, own, mt, GradByValue<S, Q>{g}, y, damping, dt, blk, part);
     else
-        fused_block<S, Q>(ps, f, own, mt, GradInMemory{f.dN}, y, damping, dt, blk, part);
+        fused_block<S, Q, LAT>(ps, f, own, mt, GradInMemory{f.dN}, y, damping, dt, blk, part);
 }
 
 template <int S, int Q>
 __global__ __launch_bounds__(kFusedThreads) void k_fused_tasks(FusedTaskArgs a, ShapeGradArg<S, Q> g) {
     __shared__ float part[kFusedIncPerBlock][9];
     const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
-    fused_dispatch<S, Q>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part);
+    fused_dispatch<S, Q, false>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part);
 }
 
-template <int S, int Q>
+template <int S, int Q, bool LAT>
 __global__ __launch_bounds__(kFusedThreads) void k_batch_fused_tasks(BatchFusedArgs b, ShapeGradArg<S, Q> g) {
-    __shared__ float part[kFusedIncPerBlock][9];
+    __shared__ float part[LAT ? kFusedIncPerBlockLat : kFusedIncPerBlock][9];
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, bid);
     const SceneDev& sd = b.scenes[seg.scene];
@@ -462,8 +478,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_batch_fused_tasks(BatchFusedA
     const MaterialView mt = sd.mt;
     const float4* y = sd.y;
     const float damping = sd.damping;
-    const int4 blk = gload(sd.fblocks, seg.first + (bid - seg.block_begin));
-    fused_dispatch<S, Q>(ps, fem, own, mt, g, y, damping, b.dt, blk, part);
+    const int4 blk = gload(LAT ? sd.fblocks_lat : sd.fblocks, seg.first + (bid - seg.block_begin));
+    fused_dispatch<S, Q, LAT>(ps, fem, own, mt, g, y, damping, b.dt, blk, part);
 }
 
 #endif
@@ -586,8 +602,12 @@ void OE_FN(launch_fused_tasks_)(const FusedTaskArgs& a, hipStream_t s) {
 
 void OE_FN(launch_batch_fused_tasks_)(const BatchFusedArgs& b, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL((k_batch_fused_tasks<OE_GATHER_S, OE_GATHER_Q>), dim3(nblocks), dim3(kFusedThreads), 0, s, b,
-                       shape_grad_arg());
+    if (b.latency)
+        hipLaunchKernelGGL((k_batch_fused_tasks<OE_GATHER_S, OE_GATHER_Q, true>), dim3(nblocks), dim3(kFusedThreads), 0,
+                           s, b, shape_grad_arg());
+    else
+        hipLaunchKernelGGL((k_batch_fused_tasks<OE_GATHER_S, OE_GATHER_Q, false>), dim3(nblocks), dim3(kFusedThreads),
+                           0, s, b, shape_grad_arg());
 }
 
 #endif

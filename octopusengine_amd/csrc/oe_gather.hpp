@@ -45,7 +45,8 @@ struct SceneDev {
     MaterialView mt;
     float4* y;
     float* ju;        // cached displacement gradients, or null
-    const int4* fblocks;   // vertex blocks of the fused element tasks, or null
+    const int4* fblocks;       // vertex blocks of the fused element tasks (throughput form), or null
+    const int4* fblocks_lat;   // the same for the latency form
     float damping;
     int iterations;
     float g[3];
@@ -85,9 +86,15 @@ struct BatchGatherArgs {
 // node r) each, kFusedThreads apart, and put the force / Hessian partial of
 // node r over all Q points of e in LDS; then one thread per slot adds its
 // vertex's partials in the owners order and runs the local solve.
+// Two forms share the kernels: the throughput form (each thread takes up to
+// kFusedRuns incidences in turn) and the latency form for launches too small
+// to fill the GPU (one incidence per thread, so a block's critical path holds
+// one element; the vertex state is loaded before the element work).  Each
+// form has its own vertex blocks.
 constexpr int kFusedThreads = 256;
 constexpr int kFusedRuns = 2;
 constexpr int kFusedIncPerBlock = kFusedThreads * kFusedRuns;
+constexpr int kFusedIncPerBlockLat = kFusedThreads;
 constexpr int kFusedMaxSlots = kFusedThreads;
 
 struct FusedTaskArgs {
@@ -111,6 +118,7 @@ struct BatchFusedArgs {
     int seg_count;
     int iteration;
     float dt;
+    int latency;   // 1: latency form and the scenes' latency vertex blocks
 };
 
 // Batched refresh of the cached displacement gradients: VSeg.count is the

@@ -288,19 +288,33 @@ GPU_VBD_FEM::~GPU_VBD_FEM() {
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
 }
 
-void GPU_VBD_FEM::prepare_tasks() {
-    if (d_fblk.get()) return;
+namespace {
+void upload_vertex_blocks(Engine* eng, const OwnersCSR& o, int max_inc, int max_slots, DBuf<int4>& d,
+                          std::vector<int32_t>& color_begin) {
     std::vector<int32_t> blocks;
     std::vector<int32_t> begin;
-    if (element == Hexa27)
-        build_vertex_blocks(owners, kQ2FusedIncPerBlock, kQ2FusedMaxSlots, blocks, begin);
-    else
-        build_vertex_blocks(owners, kFusedIncPerBlock, kFusedMaxSlots, blocks, begin);
+    build_vertex_blocks(o, max_inc, max_slots, blocks, begin);
     std::vector<int4> b4(blocks.size() / 4);
     for (size_t i = 0; i < b4.size(); ++i)
         b4[i] = make_int4(blocks[4 * i], blocks[4 * i + 1], blocks[4 * i + 2], blocks[4 * i + 3]);
-    d_fblk = DBuf<int4>(&eng->pool, b4);
-    fblk_color_begin = std::move(begin);
+    d = DBuf<int4>(&eng->pool, b4);
+    color_begin = std::move(begin);
+}
+}  // namespace
+
+void GPU_VBD_FEM::prepare_tasks() {
+    if (d_fblk.get()) return;
+    const bool q2 = element == Hexa27;
+    upload_vertex_blocks(eng, owners, q2 ? kQ2FusedIncPerBlock : kFusedIncPerBlock,
+                         q2 ? kQ2FusedMaxSlots : kFusedMaxSlots, d_fblk, fblk_color_begin);
+    try {
+        upload_vertex_blocks(eng, owners, q2 ? kQ2FusedIncPerBlockLat : kFusedIncPerBlockLat,
+                             q2 ? kQ2FusedMaxSlots : kFusedMaxSlots, d_fblk_lat, fblk_lat_color_begin);
+    } catch (const std::invalid_argument&) {
+        // a vertex with too many incidences for a latency block: throughput form only
+        d_fblk_lat = DBuf<int4>();
+        fblk_lat_color_begin.clear();
+    }
 }
 
 float* GPU_VBD_FEM::ensure_ju() {

@@ -174,6 +174,10 @@ struct GPU_VBD_FEM final : GPU_FEM {
     // prepare_tasks()
     DBuf<int4> d_fblk;
     std::vector<int32_t> fblk_color_begin;
+    // vertex blocks of the kernels' latency form (one task per thread or
+    // half-wave); absent when a vertex has more incidences than one holds
+    DBuf<int4> d_fblk_lat;
+    std::vector<int32_t> fblk_lat_color_begin;
     void prepare_tasks();
     bool tasks() const {
         return tile.mode == PASS_TASKS && version != Better_Coloration && (element != Hexa27 || has_q2_basis);

@@ -236,4 +236,21 @@ __device__ __forceinline__ void vbd_local_solve(const Acc9& a, float4 p, float4 
     }
 }
 
+// Sum of a slot's partials (part[j], j = its owners index - first_own, in
+// the owners order) and the local solve; writes the new position.
+__device__ __forceinline__ void solve_slot_from_lds(const PSView& ps, const SlotState& st, const float (*part)[9],
+                                                    int first_own, float damping, float dt) {
+    Acc9 acc;
+    acc_zero(acc);
+    for (int j = st.first_inc - first_own, e = j + st.n_inc; j < e; ++j) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
+    }
+    double dx[3];
+    vbd_local_solve(acc, st.p, st.last_p, st.y, st.m, damping, dt, dx);
+    gstore(ps.p, st.vid,
+           make_float4(st.p.x + static_cast<float>(dx[0]), st.p.y + static_cast<float>(dx[1]),
+                       st.p.z + static_cast<float>(dx[2]), st.p.w));
+}
+
 }  // namespace oe

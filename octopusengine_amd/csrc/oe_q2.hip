@@ -245,23 +245,35 @@ __device__ __forceinline__ void q2_solve_slots(const PSView& ps, const OwnersVie
                        pv.z + static_cast<float>(dx[2]), pv.w));
 }
 
+template <bool LAT>
 __device__ __forceinline__ void q2_block(const PSView& ps, const FemView& fem, const OwnersView& own,
                                          const MaterialView& mt, const q2::Basis1D& basis, const float4* y,
                                          float damping, float dt, const int4 blk) {
     __shared__ TaskLDS sh[kHalves];
-    __shared__ float part[kQ2FusedIncPerBlock][9];
+    __shared__ float part[LAT ? kQ2FusedIncPerBlockLat : kQ2FusedIncPerBlock][9];
     const int half = threadIdx.x >> 5;
-    const LanePlan P = lane_plan(basis);
-    q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
-    __syncthreads();
-    q2_solve_slots(ps, own, y, damping, dt, blk, part);
+    if constexpr (LAT) {
+        SlotState st{};
+        if (static_cast<int>(threadIdx.x) < blk.y) st = load_slot_state(ps, own, y, blk.x + threadIdx.x);
+        const LanePlan P = lane_plan(basis);
+        q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < blk.y && st.free != 0)
+            solve_slot_from_lds(ps, st, part, blk.z, damping, dt);
+    } else {
+        const LanePlan P = lane_plan(basis);
+        q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
+        __syncthreads();
+        q2_solve_slots(ps, own, y, damping, dt, blk, part);
+    }
 }
 
 __global__ __launch_bounds__(kQ2FusedThreads) void k_q2_fused(Q2FusedArgs a) {
     const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
-    q2_block(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
+    q2_block<false>(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
 }
 
+template <bool LAT>
 __global__ __launch_bounds__(kQ2FusedThreads) void k_batch_q2_fused(BatchQ2FusedArgs b) {
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, bid);
@@ -275,8 +287,8 @@ __global__ __launch_bounds__(kQ2FusedThreads) void k_batch_q2_fused(BatchQ2Fused
     const MaterialView mt = sd.mt;
     const float4* y = sd.y;
     const float damping = sd.damping;
-    const int4 blk = gload(sd.fblocks, seg.first + (bid - seg.block_begin));
-    q2_block(ps, fem, own, mt, b.basis, y, damping, b.dt, blk);
+    const int4 blk = gload(LAT ? sd.fblocks_lat : sd.fblocks, seg.first + (bid - seg.block_begin));
+    q2_block<LAT>(ps, fem, own, mt, b.basis, y, damping, b.dt, blk);
 }
 
 }  // namespace
@@ -286,7 +298,11 @@ void launch_q2_fused(const Q2FusedArgs& a, hipStream_t s) {
 }
 
 void launch_batch_q2_fused(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s) {
-    if (nblocks > 0) hipLaunchKernelGGL(k_batch_q2_fused, dim3(nblocks), dim3(kQ2FusedThreads), 0, s, b);
+    if (nblocks <= 0) return;
+    if (b.latency)
+        hipLaunchKernelGGL(k_batch_q2_fused<true>, dim3(nblocks), dim3(kQ2FusedThreads), 0, s, b);
+    else
+        hipLaunchKernelGGL(k_batch_q2_fused<false>, dim3(nblocks), dim3(kQ2FusedThreads), 0, s, b);
 }
 
 }  // namespace oe

@@ -14,8 +14,12 @@ namespace oe {
 // One block per vertex block (int4 {first slot, slot count, first owners
 // index, incidence count}, colour-major) with at most kQ2FusedIncPerBlock
 // incidences and kQ2FusedMaxSlots slots.
+// The latency form (launches too small to fill the GPU) takes one task per
+// half-wave and loads the vertex state before the element work; it has its own
+// vertex blocks of at most kQ2FusedIncPerBlockLat incidences.
 constexpr int kQ2FusedThreads = 256;
 constexpr int kQ2FusedIncPerBlock = 32;
+constexpr int kQ2FusedIncPerBlockLat = 8;
 constexpr int kQ2FusedMaxSlots = 64;
 
 struct Q2FusedArgs {
@@ -42,6 +46,7 @@ struct BatchQ2FusedArgs {
     int iteration;
     float dt;
     q2::Basis1D basis;
+    int latency;   // 1: latency form and the scenes' latency vertex blocks
 };
 
 void launch_q2_fused(const Q2FusedArgs& a, hipStream_t s);

@@ -156,7 +156,7 @@ bool Server::try_launch(std::unique_lock<std::mutex>& lk, int si) {
     launching_++;   // holds add_scene() off the table until the tick is in flight
     lk.unlock();
     // plan + launch outside the lock so that submit() is never blocked by it
-    slot.sched.build(sc, idx, tiles_);
+    slot.sched.build(sc, idx, tiles_, true);
     slot.dev.upload(eng_, slot.sched, slot.stream, false);
     launch_schedule(slot.sched, slot.dev, table, dt, sub, slot.stream, nullptr);
     OE_HIP(hipEventRecord(slot.done, slot.stream));
