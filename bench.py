@@ -37,6 +37,14 @@ from pathlib import Path
 import numpy as np
 
 ROOT = Path(__file__).resolve().parent
+
+# HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+# default).  The served leg runs 4 server streams next to the engine's own
+# stream and the null stream; with 4 queues two of them would share one and
+# serialise, so the bench asks for 8 (raising a lower setting).  It must be set
+# before the HIP runtime starts, i.e. before the engine library is used.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
@@ -360,7 +368,8 @@ def served_leg(args) -> dict:
         "value": vu / wall, "unit": "vertex-updates/s",
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_ms": float(lat.max()), "ticks": st["batches"] - base["batches"], "mean_batch": st["mean_batch"],
-        "max_batch": args.serve_max_batch, "streams": args.serve_streams, "setup_s": setup_s,
+        "max_batch": args.serve_max_batch, "streams": args.serve_streams,
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "setup_s": setup_s,
         "device_bytes": int(eng.memory()[0]), "budget_bytes": int(eng.memory()[1]),
     }
     srv.close()
@@ -397,7 +406,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson request stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="step requests per second")
     ap.add_argument("--serve-max-batch", type=int, default=64)
-    ap.add_argument("--serve-streams", type=int, default=2)
+    ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--eager-only", action="store_true",
                     help="batched leg: only the eager timed region (no graph, latency or plain loop)")
