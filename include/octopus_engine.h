@@ -168,7 +168,11 @@ oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_u
  * requests that arrive while others are on the GPU are admitted as soon as a
  * stream and their scene are free and batches overlap on the device.  Scenes
  * are admitted against the engine's memory budget when they are created
- * (oe_scene_create returns OE_ERR_BUDGET beyond it). */
+ * (oe_scene_create returns OE_ERR_BUDGET beyond it).  Each stream has its own
+ * worker thread; a tick holds scenes of one element type.  Streams only run
+ * concurrently on hardware queues of their own: keep streams + 2 (the
+ * engine's stream and the null stream) <= GPU_MAX_HW_QUEUES, whose HIP default
+ * is 4, set in the environment before the process first uses HIP. */
 oe_status oe_server_create(oe_engine* e, int32_t max_batch, int32_t streams, oe_server** out);
 oe_status oe_server_destroy(oe_server* sv);
 /* Register a scene; returns its slot id.  The scene must outlive the server. */
