@@ -4,8 +4,8 @@
 //
 // A block (256 threads, 8 half-waves) takes a run of consecutive slots of one
 // colour (a vertex block {first slot, slot count, first owners index,
-// incidence count}, at most 32 incidences).  A task is one incidence (element
-// e, local node r); half h takes the block's tasks h, h + 8, h + 16, h + 24.
+// incidence count}, at most 64 incidences).  A task is one incidence (element
+// e, local node r); half h takes the block's tasks h, h + 8, h + 16, ...
 // A task uses 27 lanes of its half (one lane per quadrature point; lanes 27-31
 // idle) and works through LDS:
 //   stage 0  lane n loads node n's displacement, stored at its tensor place
