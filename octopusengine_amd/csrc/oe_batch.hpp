@@ -27,10 +27,6 @@
 
 namespace oe {
 
-// Below this many throughput-form blocks (2 per CU) a task pass launch cannot
-// fill the GPU, and a schedule built for latency uses the latency form.
-constexpr int64_t kLatencyBlocks = 512;
-
 // Launch plan for a set of scenes.  Scene indices refer to a SceneDev table.
 // Every launch covers a contiguous run of segments whose block_begin values
 // are launch-local; kernels find their segment by binary search, so the plan

@@ -457,11 +457,11 @@ __device__ __forceinline__ void fused_dispatch(const PSView& ps, const FemView& 
         fused_block<S, Q, LAT>(ps, f, own, mt, GradInMemory{f.dN}, y, damping, dt, blk, part);
 }
 
-template <int S, int Q>
+template <int S, int Q, bool LAT>
 __global__ __launch_bounds__(kFusedThreads) void k_fused_tasks(FusedTaskArgs a, ShapeGradArg<S, Q> g) {
-    __shared__ float part[kFusedIncPerBlock][9];
+    __shared__ float part[LAT ? kFusedIncPerBlockLat : kFusedIncPerBlock][9];
     const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
-    fused_dispatch<S, Q, false>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part);
+    fused_dispatch<S, Q, LAT>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part);
 }
 
 template <int S, int Q, bool LAT>
@@ -596,8 +596,12 @@ const ShapeGradArg<OE_GATHER_S, OE_GATHER_Q>& shape_grad_arg() {
 
 void OE_FN(launch_fused_tasks_)(const FusedTaskArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
-    hipLaunchKernelGGL((k_fused_tasks<OE_GATHER_S, OE_GATHER_Q>), dim3(a.count), dim3(kFusedThreads), 0, s, a,
-                       shape_grad_arg());
+    if (a.latency)
+        hipLaunchKernelGGL((k_fused_tasks<OE_GATHER_S, OE_GATHER_Q, true>), dim3(a.count), dim3(kFusedThreads), 0, s, a,
+                           shape_grad_arg());
+    else
+        hipLaunchKernelGGL((k_fused_tasks<OE_GATHER_S, OE_GATHER_Q, false>), dim3(a.count), dim3(kFusedThreads), 0, s,
+                           a, shape_grad_arg());
 }
 
 void OE_FN(launch_batch_fused_tasks_)(const BatchFusedArgs& b, int nblocks, hipStream_t s) {

@@ -96,6 +96,10 @@ constexpr int kFusedRuns = 2;
 constexpr int kFusedIncPerBlock = kFusedThreads * kFusedRuns;
 constexpr int kFusedIncPerBlockLat = kFusedThreads;
 constexpr int kFusedMaxSlots = kFusedThreads;
+// Below this many throughput-form blocks (2 per CU) a task-pass launch cannot
+// fill the GPU: single scenes, and batches built for latency, use the latency
+// form there.
+constexpr int64_t kLatencyBlocks = 512;
 
 struct FusedTaskArgs {
     PSView ps;
@@ -103,11 +107,12 @@ struct FusedTaskArgs {
     OwnersView own;
     MaterialView mt;
     const float4* y;
-    const int4* blocks;   // vertex blocks of the scene, colour-major
+    const int4* blocks;   // vertex blocks of the scene, colour-major (of the form used)
     float damping;
     float dt;
     int first_block;      // the colour's vertex blocks: [first_block, first_block + count)
     int count;
+    int latency;          // 1: latency form (blocks are the latency-form vertex blocks)
 };
 // Batched: BatchSeg.first / count are a range of one scene's vertex blocks,
 // one block each.

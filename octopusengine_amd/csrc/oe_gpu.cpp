@@ -398,16 +398,19 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
             }
             OE_HIP(hipEventRecord(ev_pool[ev_used], s));
         }
-        if (by_tasks && element == Hexa27) {
-            const int b0 = fblk_color_begin[c];
-            launch_q2_fused(Q2FusedArgs{a.ps, a.fem, a.own, a.mt, q2_basis, y, d_fblk.get(), damping, dt, b0,
-                                        fblk_color_begin[c + 1] - b0},
-                            s);
-        } else if (by_tasks) {
-            const int b0 = fblk_color_begin[c];
-            if (!launch_fused_tasks(S, Q, FusedTaskArgs{a.ps, a.fem, a.own, a.mt, y, d_fblk.get(), damping, dt, b0,
-                                                        fblk_color_begin[c + 1] - b0},
-                                    s))
+        if (by_tasks) {
+            // the latency form for colours under 2 throughput-form blocks per CU
+            const bool lat = d_fblk_lat.get() && fblk_color_begin[c + 1] - fblk_color_begin[c] < kLatencyBlocks;
+            const std::vector<int32_t>& cb = lat ? fblk_lat_color_begin : fblk_color_begin;
+            const int4* blocks = lat ? d_fblk_lat.get() : d_fblk.get();
+            const int b0 = cb[c], nb = cb[c + 1] - cb[c];
+            if (element == Hexa27)
+                launch_q2_fused(Q2FusedArgs{a.ps, a.fem, a.own, a.mt, q2_basis, y, blocks, damping, dt, b0, nb,
+                                            lat ? 1 : 0},
+                                s);
+            else if (!launch_fused_tasks(S, Q, FusedTaskArgs{a.ps, a.fem, a.own, a.mt, y, blocks, damping, dt, b0, nb,
+                                                             lat ? 1 : 0},
+                                         s))
                 throw std::invalid_argument("unsupported element");
         } else if (!launch_gather(S, Q, tile.L, tile.QL, a, s)) {
             throw std::invalid_argument("unsupported gather tile shape");

@@ -4,7 +4,7 @@
 //
 // A block (256 threads, 8 half-waves) takes a run of consecutive slots of one
 // colour (a vertex block {first slot, slot count, first owners index,
-// incidence count}, at most 64 incidences).  A task is one incidence (element
+// incidence count}, at most 32 incidences).  A task is one incidence (element
 // e, local node r); half h takes the block's tasks h, h + 8, h + 16, ...
 // A task uses 27 lanes of its half (one lane per quadrature point; lanes 27-31
 // idle) and works through LDS:
@@ -268,9 +268,10 @@ __device__ __forceinline__ void q2_block(const PSView& ps, const FemView& fem, c
     }
 }
 
+template <bool LAT>
 __global__ __launch_bounds__(kQ2FusedThreads) void k_q2_fused(Q2FusedArgs a) {
     const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
-    q2_block<false>(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
+    q2_block<LAT>(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
 }
 
 template <bool LAT>
@@ -294,7 +295,11 @@ __global__ __launch_bounds__(kQ2FusedThreads) void k_batch_q2_fused(BatchQ2Fused
 }  // namespace
 
 void launch_q2_fused(const Q2FusedArgs& a, hipStream_t s) {
-    if (a.count > 0) hipLaunchKernelGGL(k_q2_fused, dim3(a.count), dim3(kQ2FusedThreads), 0, s, a);
+    if (a.count <= 0) return;
+    if (a.latency)
+        hipLaunchKernelGGL(k_q2_fused<true>, dim3(a.count), dim3(kQ2FusedThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_q2_fused<false>, dim3(a.count), dim3(kQ2FusedThreads), 0, s, a);
 }
 
 void launch_batch_q2_fused(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s) {

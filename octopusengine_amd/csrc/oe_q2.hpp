@@ -18,7 +18,9 @@ namespace oe {
 // half-wave and loads the vertex state before the element work; it has its own
 // vertex blocks of at most kQ2FusedIncPerBlockLat incidences.
 constexpr int kQ2FusedThreads = 256;
-constexpr int kQ2FusedIncPerBlock = 64;   // 8 tasks per half-wave (4: 2% slower, 16: 4% slower)
+// 4 tasks per half-wave.  8 are 2% faster on a large batch but double the
+// critical path of a block, which the served latency of large scenes pays.
+constexpr int kQ2FusedIncPerBlock = 32;
 constexpr int kQ2FusedIncPerBlockLat = 8;
 constexpr int kQ2FusedMaxSlots = 64;
 
@@ -29,11 +31,12 @@ struct Q2FusedArgs {
     MaterialView mt;
     q2::Basis1D basis;
     const float4* y;
-    const int4* blocks;   // vertex blocks of the scene
+    const int4* blocks;   // vertex blocks of the scene (of the form used)
     float damping;
     float dt;
     int first_block;      // the colour's blocks: [first_block, first_block + count)
     int count;
+    int latency;          // 1: latency form (blocks are the latency-form vertex blocks)
 };
 
 // Batched: BatchSeg.first / count are a range of one scene's vertex blocks,
