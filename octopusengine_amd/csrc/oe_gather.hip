@@ -298,9 +298,8 @@ __global__ __launch_bounds__(256) void k_batch_ju_refresh(BatchRefreshArgs b) {
 
 #if !defined(OE_GATHER_S) || OE_GATHER_S != 27   // 27-node hexahedra have their own task kernels
 // Fused element tasks (see FusedTaskArgs).  Phase 1: thread t of the block
-// takes the incidences t, t + kFusedThreads, ... of the vertex block; the
-// next incidence's owners entry and node indices are loaded while the current
-// one runs.  Fixed vertices are not skipped there: phase 2 skips them.
+// takes the incidences t, t + kFusedThreads, ... of the vertex block.  Fixed
+// vertices are not skipped there: phase 2 skips them.
 //
 // The shape-derivative table is read through an accessor.  Tables of up to
 // 3 KB travel by value in the kernel arguments, so the Jx contraction (the
@@ -384,25 +383,16 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
     const int first_own = blk.z, n_inc = blk.w;
     const int t0 = threadIdx.x;
     if (t0 >= n_inc) return;
-    const int last = first_own + n_inc - 1;
-    int pk = gload(own.own, first_own + t0);
-    int tp[S];
-#pragma unroll
-    for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
-    int pk_next = gload(own.own, min(first_own + t0 + kFusedThreads, last));
 #pragma unroll 1
     for (int t = t0; t < n_inc; t += kFusedThreads) {
+        const int pk = gload(own.own, first_own + t);
         float xs[S][3];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            const float4 x = gload(static_cast<const float4*>(ps.p), tp[k]);
+            const float4 x = gload(static_cast<const float4*>(ps.p), gload(f.topo, static_cast<size_t>(pk >> 5) * S + k));
             xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
         }
         const int eid = pk >> 5, r = pk & 31;
-        pk = pk_next;
-#pragma unroll
-        for (int k = 0; k < S; ++k) tp[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
-        pk_next = gload(own.own, min(first_own + t + 2 * kFusedThreads, last));
         Acc9 acc;
         elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);
 #pragma unroll

@@ -90,9 +90,12 @@ struct BatchGatherArgs {
 // kFusedRuns incidences in turn) and the latency form for launches too small
 // to fill the GPU (one incidence per thread, so a block's critical path holds
 // one element; the vertex state is loaded before the element work).  Each
-// form has its own vertex blocks.
+// form has its own vertex blocks.  One run per thread measured fastest for the
+// throughput form too (P2 batch: 11.25 ms per step against 11.50 ms for two,
+// with no loads run ahead: those held 12 VGPRs and cost a wave per SIMD), so
+// the two forms differ only in when the vertex state is loaded.
 constexpr int kFusedThreads = 256;
-constexpr int kFusedRuns = 2;
+constexpr int kFusedRuns = 1;
 constexpr int kFusedIncPerBlock = kFusedThreads * kFusedRuns;
 constexpr int kFusedIncPerBlockLat = kFusedThreads;
 constexpr int kFusedMaxSlots = kFusedThreads;

@@ -1,5 +1,5 @@
 """Element-task vertex blocks at their limits: a vertex with more incident
-elements than a task block holds (512 for tetrahedra, 32 for 27-node
+elements than a task block holds (256 for tetrahedra, 32 for 27-node
 hexahedra).  Explicit task tiles are refused; the default pass falls back to
 the gather kernel; a vertex just under the limit runs in one block.  Results
 are compared with the FP64 oracle (max vertex error <= 1e-5 of the bbox
