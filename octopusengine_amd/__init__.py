@@ -360,8 +360,8 @@ class Scene:
 
     def set_tile(self, lanes: int, qpl: int, mode: int = PASS_GATHER):
         """VBD colour-pass layout: lanes per vertex, quadrature points per lane
-        and mode (PASS_GATHER, PASS_CACHE: cached gradients with qpl = 1,
-        PASS_TASKS: element tasks, 27-node hexahedra)."""
+        and mode (PASS_GATHER; PASS_CACHE: cached gradients with qpl = 1;
+        PASS_TASKS: element tasks, which ignore lanes and qpl)."""
         L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl, int(mode)))
 
     def get_tile(self):
