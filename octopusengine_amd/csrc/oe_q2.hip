@@ -9,14 +9,17 @@
 // A task uses 27 lanes of its half (one lane per quadrature point; lanes 27-31
 // idle) and works through LDS:
 //   stage 0  lane n loads node n's displacement, stored at its tensor place
-//   stage 1  lane (a, b, k):  T  = sum_c l_c(z_k)  u_abc,  Tz = sum_c l_c'(z_k) u_abc
-//   stage 2  lane (a, j, k):  X  = sum_b l_b(y_j) T_ab,  Xy = sum_b l_b'(y_j) T_ab,
-//                             Xz = sum_b l_b(y_j) Tz_ab
-//   stage 3  lane (i, j, k):  dU/dx = sum_a l_a'(x_i) X_a, dU/dy = sum_a l_a(x_i) Xy_a,
+//   stage 1  lane (a, j, k):  with T_b = sum_c l_c(z_k) u_abc, Tz_b = sum_c l_c'(z_k) u_abc,
+//                             X = sum_b l_b(y_j) T_b,  Xy = sum_b l_b'(y_j) T_b,
+//                             Xz = sum_b l_b(y_j) Tz_b
+//   stage 2  lane (i, j, k):  dU/dx = sum_a l_a'(x_i) X_a, dU/dy = sum_a l_a(x_i) Xy_a,
 //                             dU/dz = sum_a l_a(x_i) Xz_a
-// - 72 FMAs per point and coordinate triple instead of the 243 of the plain sum
-// over 27 nodes, one load per node instead of one per (node, point) - then the
-// point's force / Hessian contribution to node r.  The half's lanes add their
+// - 108 FMAs per point and coordinate triple instead of the 243 of the plain
+// sum over 27 nodes, one load per node instead of one per (node, point) - then
+// the point's force / Hessian contribution to node r.  The pass is bound by
+// each half's chain of dependent LDS round trips, not by its arithmetic: the
+// two contractions of stage 1 share one stage (36 more FMAs than a stage each,
+// one wait less), and u and x have their own LDS rows (one more wait less).  The half's lanes add their
 // contributions with DPP and the task's 9-float partial goes to LDS.  After a
 // block barrier one thread per slot adds its vertex's partials in the owners
 // order and runs the FP64 local solve, the arithmetic of the gather kernel's
@@ -76,13 +79,11 @@ __device__ __forceinline__ Basis3 basis_at(const q2::Basis1D& B, int i) {
 }
 
 // One half's LDS.  Rows 27-31 are written by the idle lanes and never read.
-// x reuses u's space: u is dead once stage 1 has read it, x once stage 3 has.
+// u and x do not overlap, so a task's stage 0 may follow the previous task's
+// stage 2 without a wait: a wave's LDS operations are performed in order.
 struct TaskLDS {
-    float4 t[32][2];   // (a, b, k) -> T, Tz
-    union {
-        float4 u[32];      // tensor place -> displacement
-        float4 x[32][3];   // (a, j, k) -> X, Xy, Xz
-    };
+    float4 u[32];      // tensor place -> displacement
+    float4 x[32][3];   // (a, j, k) -> X, Xy, Xz
 };
 
 // x + (x at lane ctrl(lane)) for the rows in ROWS; other rows add 0
@@ -107,7 +108,7 @@ __device__ __forceinline__ float half_sum_to_upper(float x) {
 // factors of the three stages and its LDS rows.
 struct LanePlan {
     int l, node, place, q;
-    int ab3, b0, x0;   // stage-1 u rows, stage-2 t rows, stage-3 x rows
+    int a9, x0;   // stage-1 u rows a * 9 + ..., stage-2 x rows x0 + 9 a
     Basis3 s1, s2, s3;
 };
 __device__ __forceinline__ LanePlan lane_plan(const q2::Basis1D& B) {
@@ -118,8 +119,7 @@ __device__ __forceinline__ LanePlan lane_plan(const q2::Basis1D& B) {
     p.node = lc;
     p.place = kQ2.place[lc];
     p.q = kQ2.node[lc];
-    p.ab3 = (p0 * 3 + p1) * 3;
-    p.b0 = p0 * 9 + p2;
+    p.a9 = p0 * 9;
     p.x0 = p1 * 3 + p2;
     p.s1 = basis_at(B, p2);
     p.s2 = basis_at(B, p1);
@@ -140,30 +140,34 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
     // stage 0
     sh.u[P.place] = u;
     half_sync();
-    // stage 1: (a, b, k)
+    // stage 1: lane (a, j, k); u_abc sits at row a * 9 + b * 3 + c
     {
-        const float4 u0 = sh.u[P.ab3], u1 = sh.u[P.ab3 + 1], u2 = sh.u[P.ab3 + 2];
-        const Basis3& b = P.s1;
-        sh.t[P.l][0] = make_float4(b.l0 * u0.x + b.l1 * u1.x + b.l2 * u2.x, b.l0 * u0.y + b.l1 * u1.y + b.l2 * u2.y,
-                                   b.l0 * u0.z + b.l1 * u1.z + b.l2 * u2.z, 0.f);
-        sh.t[P.l][1] = make_float4(b.d0 * u0.x + b.d1 * u1.x + b.d2 * u2.x, b.d0 * u0.y + b.d1 * u1.y + b.d2 * u2.y,
-                                   b.d0 * u0.z + b.d1 * u1.z + b.d2 * u2.z, 0.f);
+        const Basis3& bz = P.s1;
+        const Basis3& by = P.s2;
+        float X[3] = {0.f, 0.f, 0.f}, Xy[3] = {0.f, 0.f, 0.f}, Xz[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float4 u0 = sh.u[P.a9 + b * 3], u1 = sh.u[P.a9 + b * 3 + 1], u2 = sh.u[P.a9 + b * 3 + 2];
+            const float T[3] = {bz.l0 * u0.x + bz.l1 * u1.x + bz.l2 * u2.x, bz.l0 * u0.y + bz.l1 * u1.y + bz.l2 * u2.y,
+                                bz.l0 * u0.z + bz.l1 * u1.z + bz.l2 * u2.z};
+            const float Tz[3] = {bz.d0 * u0.x + bz.d1 * u1.x + bz.d2 * u2.x,
+                                 bz.d0 * u0.y + bz.d1 * u1.y + bz.d2 * u2.y,
+                                 bz.d0 * u0.z + bz.d1 * u1.z + bz.d2 * u2.z};
+            const float lb = b == 0 ? by.l0 : (b == 1 ? by.l1 : by.l2);
+            const float db = b == 0 ? by.d0 : (b == 1 ? by.d1 : by.d2);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[c] += lb * T[c];
+                Xy[c] += db * T[c];
+                Xz[c] += lb * Tz[c];
+            }
+        }
+        sh.x[P.l][0] = make_float4(X[0], X[1], X[2], 0.f);
+        sh.x[P.l][1] = make_float4(Xy[0], Xy[1], Xy[2], 0.f);
+        sh.x[P.l][2] = make_float4(Xz[0], Xz[1], Xz[2], 0.f);
     }
     half_sync();
-    // stage 2: (a, j, k); T of (a, b, k) sits at row a * 9 + b * 3 + k
-    {
-        const float4 t0 = sh.t[P.b0][0], t1 = sh.t[P.b0 + 3][0], t2 = sh.t[P.b0 + 6][0];
-        const float4 z0 = sh.t[P.b0][1], z1 = sh.t[P.b0 + 3][1], z2 = sh.t[P.b0 + 6][1];
-        const Basis3& b = P.s2;
-        sh.x[P.l][0] = make_float4(b.l0 * t0.x + b.l1 * t1.x + b.l2 * t2.x, b.l0 * t0.y + b.l1 * t1.y + b.l2 * t2.y,
-                                   b.l0 * t0.z + b.l1 * t1.z + b.l2 * t2.z, 0.f);
-        sh.x[P.l][1] = make_float4(b.d0 * t0.x + b.d1 * t1.x + b.d2 * t2.x, b.d0 * t0.y + b.d1 * t1.y + b.d2 * t2.y,
-                                   b.d0 * t0.z + b.d1 * t1.z + b.d2 * t2.z, 0.f);
-        sh.x[P.l][2] = make_float4(b.l0 * z0.x + b.l1 * z1.x + b.l2 * z2.x, b.l0 * z0.y + b.l1 * z1.y + b.l2 * z2.y,
-                                   b.l0 * z0.z + b.l1 * z1.z + b.l2 * z2.z, 0.f);
-    }
-    half_sync();
-    // stage 3: point (i, j, k); X of (a, j, k) sits at row a * 9 + j * 3 + k
+    // stage 2: point (i, j, k); X of (a, j, k) sits at row a * 9 + j * 3 + k
     float Ju[9];   // column-major: Ju[c * 3 + comp] = d u_comp / d xi_c
     {
         const float4 xa = sh.x[P.x0][0], xb = sh.x[P.x0 + 9][0], xc = sh.x[P.x0 + 18][0];
@@ -180,8 +184,6 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
         Ju[7] = b.l0 * za.y + b.l1 * zb.y + b.l2 * zc.y;
         Ju[8] = b.l0 * za.z + b.l1 * zb.z + b.l2 * zc.z;
     }
-    // stage 3 has read x before the next task's stage 0 overwrites it (u shares x's space)
-    half_sync();
     nh_contrib(Ju, JXi, V, dnr, mt, acc);
 }
 
