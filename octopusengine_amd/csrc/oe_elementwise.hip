@@ -128,12 +128,14 @@ __global__ __launch_bounds__(kBlock) void k_fix(PSView ps, const int* ids, int n
 }
 
 // clamp the absolute position X + u into [pmin, pmax]; u is rewritten only
-// when a component is clamped
-__device__ __forceinline__ void box_clamp(float4& u, float4 X, const float pmin[3], const float pmax[3]) {
+// when a component is clamped, and the return value says whether one was
+__device__ __forceinline__ bool box_clamp(float4& u, float4 X, const float pmin[3], const float pmax[3]) {
     const float x = X.x + u.x, y = X.y + u.y, z = X.z + u.z;
-    if (x < pmin[0] || x > pmax[0]) u.x = fminf(fmaxf(x, pmin[0]), pmax[0]) - X.x;
-    if (y < pmin[1] || y > pmax[1]) u.y = fminf(fmaxf(y, pmin[1]), pmax[1]) - X.y;
-    if (z < pmin[2] || z > pmax[2]) u.z = fminf(fmaxf(z, pmin[2]), pmax[2]) - X.z;
+    const bool cx = x < pmin[0] || x > pmax[0], cy = y < pmin[1] || y > pmax[1], cz = z < pmin[2] || z > pmax[2];
+    if (cx) u.x = fminf(fmaxf(x, pmin[0]), pmax[0]) - X.x;
+    if (cy) u.y = fminf(fmaxf(y, pmin[1]), pmax[1]) - X.y;
+    if (cz) u.z = fminf(fmaxf(z, pmin[2]), pmax[2]) - X.z;
+    return cx || cy || cz;
 }
 
 __global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 pmax) {
@@ -142,8 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 p
     if (ps.mask[i] == 0) return;
     float4 u = ps.p[i];
     const float lo[3] = {pmin.x, pmin.y, pmin.z}, hi[3] = {pmax.x, pmax.y, pmax.z};
-    box_clamp(u, ps.init_p[i], lo, hi);
-    ps.p[i] = u;
+    if (box_clamp(u, ps.init_p[i], lo, hi)) ps.p[i] = u;   // written only when clamped
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* slot_vid, int first, int count,
@@ -204,8 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fix_box(BatchVArgs b) {
     }
     if (!sd.has_box || ps.mask[i] == 0) return;
     float4 u = ps.p[i];
-    box_clamp(u, ps.init_p[i], sd.box_min, sd.box_max);
-    ps.p[i] = u;
+    if (box_clamp(u, ps.init_p[i], sd.box_min, sd.box_max)) ps.p[i] = u;   // written only when clamped
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_velocity(BatchVArgs b) {

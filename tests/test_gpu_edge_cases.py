@@ -101,3 +101,28 @@ def test_server_drain_when_idle_and_back_to_back_requests(engine):
     assert st["completed"] == 4 and st["batches"] == 4
     assert err <= 1e-6
     srv.close()
+
+
+@pytest.mark.parametrize("el", [oe.TETRA10, oe.HEXA27])
+def test_box_clamps_a_falling_beam(engine, el):
+    """A free beam falls onto the box floor (y >= -0.02): the box clamps its
+    lowest vertices in every iteration of the later steps, alone and batched."""
+    lo, hi = (-10.0, -0.02, -10.0), (10.0, 5.0, 10.0)
+    g, o, rest, _ = make_pair(engine, el, (4, 1, 1), fix=False, box=False)
+    b_scene, _, _, _ = make_pair(engine, el, (4, 1, 1), fix=False, box=False)
+    for s in (g, b_scene):
+        s.box_limit(lo, hi)
+    o.box(lo, hi)
+    batch = oe.Batch(engine, [b_scene])
+    for _ in range(20):
+        g.step(0.005, 1)
+        batch.step(0.005, 1)
+        o.step(0.005, 1)
+    y_min = float(o.positions()[:, 1].min())
+    err = max_err(g.positions(), o.positions())
+    err_b = max_err(b_scene.positions(), o.positions())
+    print(f"falling beam el={el}: lowest y {y_min:.5f}, err alone {err:.3e}, batched {err_b:.3e}", flush=True)
+    assert abs(y_min - lo[1]) <= 1e-6          # the floor was reached and holds
+    assert float(g.positions()[:, 1].min()) >= lo[1] - 1e-6
+    assert err <= TOL and err_b <= TOL
+    batch.close()
