@@ -70,7 +70,7 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
     }
     d.ps = s.ps->get_parameters();
     d.fem = s.fem->fem_view();
-    d.own = s.vbd_fem->owners_view();
+    d.own = OwnersView{s.vbd_fem->d_slot_vid.get(), s.vbd_fem->d_slot_nbo.get(), s.vbd_fem->d_own.get()};
     d.mt = s.fem->mt;
     auto* vbd = dynamic_cast<GPU_VBD*>(s.integrator.get());
     if (!vbd) throw std::logic_error("VBD scene without a GPU_VBD integrator");

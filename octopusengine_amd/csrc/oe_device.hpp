@@ -209,13 +209,7 @@ struct OwnersView {
     const int* slot_vid;
     const int2* slot_nbo;   // (incident element count, offset into own)
     const int* own;         // (eid << 5) | local node
-    // incidence-major node ids: row j (inc_stride(S) ints) holds the S node
-    // ids of own[j]'s element, so that an incidence's nodes are aligned
-    // 16-byte loads next to those of its neighbours; null when not built
-    // (only the fused element tasks read it)
-    const int* inc_nodes;
 };
-constexpr int inc_stride(int S) { return (S + 3) & ~3; }
 
 // What the vertex solve of one slot reads.  The latency forms of the fused
 // task kernels load it before the element work of their block, so that the

@@ -386,20 +386,10 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
 #pragma unroll 1
     for (int t = t0; t < n_inc; t += kFusedThreads) {
         const int pk = gload(own.own, first_own + t);
-        // the incidence's node ids: its row of the incidence-major table,
-        // aligned 16-byte loads issued together with the owners entry
-        constexpr int W = inc_stride(S);
-        int nodes[W];
-        const int4* row = reinterpret_cast<const int4*>(own.inc_nodes) + static_cast<size_t>(first_own + t) * (W / 4);
-#pragma unroll
-        for (int i = 0; i < W / 4; ++i) {
-            const int4 v = gload(row, i);
-            nodes[4 * i] = v.x; nodes[4 * i + 1] = v.y; nodes[4 * i + 2] = v.z; nodes[4 * i + 3] = v.w;
-        }
         float xs[S][3];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            const float4 x = gload(static_cast<const float4*>(ps.p), nodes[k]);
+            const float4 x = gload(static_cast<const float4*>(ps.p), gload(f.topo, static_cast<size_t>(pk >> 5) * S + k));
             xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
         }
         const int eid = pk >> 5, r = pk & 31;

@@ -305,17 +305,6 @@ void upload_vertex_blocks(Engine* eng, const OwnersCSR& o, int max_inc, int max_
 void GPU_VBD_FEM::prepare_tasks() {
     if (d_fblk.get()) return;
     const bool q2 = element == Hexa27;
-    if (!q2) {
-        // incidence-major node ids (the Q2 kernel reads an element's row of
-        // the topology with 27 lanes, already contiguous)
-        std::vector<int> t;
-        topo.download(t, eng->stream);
-        const int w = inc_stride(S);
-        std::vector<int> rows(owners.own.size() * static_cast<size_t>(w), 0);
-        for (size_t j = 0; j < owners.own.size(); ++j)
-            std::copy_n(&t[static_cast<size_t>(owners.own[j] >> 5) * S], S, &rows[j * w]);
-        d_inc_nodes = DBuf<int>(&eng->pool, rows);
-    }
     upload_vertex_blocks(eng, owners, q2 ? kQ2FusedIncPerBlock : kFusedIncPerBlock,
                          q2 ? kQ2FusedMaxSlots : kFusedMaxSlots, d_fblk, fblk_color_begin);
     try {
@@ -344,7 +333,7 @@ void GPU_VBD_FEM::residual(GPU_ParticleSystem* ps, float dt, std::vector<float>&
     GatherArgs a{};
     a.ps = ps->get_parameters();
     a.fem = fem_view();
-    a.own = owners_view();
+    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
     a.mt = mt;
     a.y = y;
     a.p_new = d_residual.get();
@@ -380,7 +369,7 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
     GatherArgs a{};
     a.ps = ps->get_parameters();
     a.fem = fem_view();
-    a.own = owners_view();
+    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
     a.mt = mt;
     a.y = y;
     a.p_new = p_new.get();
@@ -491,7 +480,7 @@ void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
     GatherArgs a{};
     a.ps = ps->get_parameters();
     a.fem = fem_view();
-    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get(), nullptr};
+    a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
     a.mt = mt;
     a.damping = _damping;
     a.dt = dt;

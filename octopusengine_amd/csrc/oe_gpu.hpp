@@ -178,11 +178,6 @@ struct GPU_VBD_FEM final : GPU_FEM {
     // half-wave); absent when a vertex has more incidences than one holds
     DBuf<int4> d_fblk_lat;
     std::vector<int32_t> fblk_lat_color_begin;
-    // incidence-major node ids of the fused element tasks (all types but Q2)
-    DBuf<int> d_inc_nodes;
-    OwnersView owners_view() const {
-        return OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get(), d_inc_nodes.get()};
-    }
     void prepare_tasks();
     bool tasks() const {
         return tile.mode == PASS_TASKS && version != Better_Coloration && (element != Hexa27 || has_q2_basis);
