@@ -277,9 +277,9 @@ __global__ __launch_bounds__(kQ2FusedThreads, LAT ? 1 : 6) void k_q2_fused(Q2Fus
 }
 
 // The throughput form is held to 80 VGPRs (6 waves per SIMD).  The compiler
-// then keeps 4 dwords in scratch, and the batch still runs 2.4% faster than at
-// 5 waves (41.4 against 42.4 ms per step on the Q2 scenes of the 256-scene
-// batch).  The latency form keeps its registers (no spill).
+// then keeps 2 dwords in scratch, and the 256-scene batch still steps 3%
+// faster than at 5 waves, where the kernel needs 82 VGPRs and spills nothing
+// (50.6 against 52.2 ms per step).  The latency form keeps its registers.
 template <bool LAT>
 __global__ __launch_bounds__(kQ2FusedThreads, LAT ? 1 : 6) void k_batch_q2_fused(BatchQ2FusedArgs b) {
     const int bid = xcd_block(blockIdx.x, gridDim.x);
