@@ -73,7 +73,7 @@ def cpu_info() -> tuple[str, int]:
     return model, os.cpu_count() or 0
 
 
-def config4_specs(n: int = 256, seed: int = 4):
+def batched_beam_specs(n: int = 256, seed: int = 4):
     """(element, cells, E, nu) of the batched workload, drawn from `seed`."""
     import octopusengine_amd as oe
 
@@ -132,7 +132,7 @@ def measured_traffic(kernel_prefix: str, workload: str):
 def batched_leg(args, eng) -> dict:
     import octopusengine_amd as oe
 
-    specs = config4_specs(args.scenes, args.seed)
+    specs = batched_beam_specs(args.scenes, args.seed)
     t0 = time.perf_counter()
     scenes = build_scenes(eng, specs, args.workers)
     setup_s = time.perf_counter() - t0

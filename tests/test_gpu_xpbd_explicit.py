@@ -26,7 +26,7 @@ def test_xpbd_one_frame(engine, case):
     el, cells, version = case
     g, o, rest, _ = make_pair(engine, el, cells, solver=1, material=2, young=1e6, poisson=0.45,
                               xpbd_version=version)
-    # one frame: dt = 0.01 with 50 substeps (config 3 XPBD)
+    # one frame: dt = 0.01 with 50 substeps, as the XPBD P3/Q2 workload steps
     g.step(0.01, 50)
     o.step(0.01, 50)
     err = max_err(g.positions(), o.positions())
@@ -72,7 +72,7 @@ EXPLICIT_CASES = [(9, (4, 2, 2)), (8, (3, 1, 1)), (7, (4, 2, 2)), (3, (8, 2, 2))
 @pytest.mark.parametrize("case", EXPLICIT_CASES, ids=[f"el{c[0]}" for c in EXPLICIT_CASES])
 def test_explicit_substeps(engine, case):
     el, cells = case
-    # config 3: NeoHooke E=1e6 nu=0.45, +-1e-3 m noise, dt = 1e-5
+    # the explicit P3/Q2 workload: NeoHooke E=1e6 nu=0.45, +-1e-3 m noise, dt = 1e-5
     g, o, rest, _ = make_pair(engine, el, cells, solver=2, material=2, young=1e6, poisson=0.45, noise=1e-3, seed=7)
     g.step(2e-3, 200)      # 200 substeps of 1e-5
     o.step(2e-3, 200)

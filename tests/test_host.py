@@ -35,7 +35,7 @@ def test_mesh_bit_exact(el, cells):
     assert np.array_equal(x1.view(np.int32), x2.view(np.int32))
 
 
-def test_config_mesh_sizes():
+def test_beam_mesh_sizes():
     assert oe.beam_mesh(oe.TETRA, (16, 4, 4))[0].shape == (425, 3)
     xyz, topo = oe.beam_mesh(oe.TETRA10, (64, 16, 16))
     assert xyz.shape == (140481, 3) and topo.shape == (98304, 10)
@@ -54,7 +54,7 @@ def test_colorings_bit_exact(el, cells):
     assert np.array_equal(d1, d2) and m1 == m2 and k2 == 0
 
 
-def test_config2_coloring_bit_exact():
+def test_p2_64x16x16_coloring_bit_exact():
     xyz, topo = oe.beam_mesh(oe.TETRA10, (64, 16, 16))
     c1, n1, k1 = oe.host_color_primal(oe.TETRA10, topo, xyz.shape[0])
     c2, n2 = O.color_dsat(oe.TETRA10, topo, xyz.shape[0])
