@@ -196,16 +196,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_fix_box(BatchVArgs b) {
     const SceneDev& sd = vseg_scene(b, i, ok);
     if (!ok || b.iteration >= sd.iterations) return;
     const PSView ps = sd.ps;
-    if (sd.fixed && sd.fixed[i]) {
-        ps.p[i] = fix_displacement(ps.init_p[i], sd.fix_com, sd.fix_offset, sd.fix_R, ps.p[i].w);
-        ps.v[i] = f4(0.f, 0.f, 0.f, 0.f);
-        ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
-        ps.mask[i] = 0;
+    const unsigned char* fixed = sd.fixed;
+    // every load at once (global address space): the flag, the mask, the
+    // displacement and the rest position
+    const bool is_fixed = fixed != nullptr && gload(fixed, i) != 0;
+    const int mask = gload(static_cast<const int*>(ps.mask), i);
+    float4 u = gload(static_cast<const float4*>(ps.p), i);
+    const float4 X = gload(static_cast<const float4*>(ps.init_p), i);
+    if (is_fixed) {
+        gstore(ps.p, i, fix_displacement(X, sd.fix_com, sd.fix_offset, sd.fix_R, u.w));
+        gstore(ps.v, i, f4(0.f, 0.f, 0.f, 0.f));
+        gstore(ps.f, i, f4(0.f, 0.f, 0.f, 0.f));
+        gstore(ps.mask, i, 0);
         return;
     }
-    if (!sd.has_box || ps.mask[i] == 0) return;
-    float4 u = ps.p[i];
-    if (box_clamp(u, ps.init_p[i], sd.box_min, sd.box_max)) ps.p[i] = u;   // written only when clamped
+    if (!sd.has_box || mask == 0) return;
+    if (box_clamp(u, X, sd.box_min, sd.box_max)) gstore(ps.p, i, u);   // written only when clamped
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_velocity(BatchVArgs b) {
