@@ -19,11 +19,11 @@
 // the point's force / Hessian contribution to node r.  The pass is bound by
 // each half's chain of dependent LDS round trips, not by its arithmetic: the
 // two contractions of stage 1 share one stage (36 more FMAs than a stage each,
-// one wait less), and u and x have their own LDS rows (one more wait less).  The half's lanes add their
-// contributions with DPP and the task's 9-float partial goes to LDS.  After a
-// block barrier one thread per slot adds its vertex's partials in the owners
-// order and runs the FP64 local solve, the arithmetic of the gather kernel's
-// final lane.
+// one wait less), and u and x have their own LDS rows (one more wait less).
+// The half's lanes add their contributions with DPP and the task's 9-float
+// partial goes to LDS.  After a block barrier one thread per slot adds its
+// vertex's partials in the owners order and runs the FP64 local solve, the
+// arithmetic of the gather kernel's final lane.
 #include "oe_kernels.hpp"
 #include "oe_q2.hpp"
 
