@@ -1,29 +1,43 @@
 /*
  * FP64 CPU oracle - TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
- * How each part is pinned (tests/test_oracle.py unless stated):
- *   shapes        Kronecker delta at the nodes, partition of unity, exact
+ * How each part is pinned (tests/test_oracle.py = O, tests/test_oracle_pins.py
+ * = P, tests/test_golden_vtk.py = G):
+ *   shapes        O: Kronecker delta at the nodes, partition of unity, exact
  *                 derivatives vs finite differences, quadrature exactness on
  *                 monomials; cross-checked against the product's independent
- *                 Vandermonde derivation.
- *   beam / conv.  closed-form counts (425/1536, 140481, 18785, 60625), grid
+ *                 Vandermonde derivation (tests/test_host.py).
+ *   beam / conv.  O: closed-form counts (425/1536, 140481, 18785, 60625), grid
  *                 coordinates, node = edge midpoint / third / face centroid,
  *                 signed volumes summing to the beam volume, first-encounter
- *                 numbering on a hand-checked 1-cell case.
- *   DSAT / SLF    hand-traced known answers on small graphs (K4, two tets
+ *                 numbering on a hand-checked 1-cell case.  G: the reference's
+ *                 own result files (mesh sections, masses).
+ *   DSAT / SLF    O: hand-traced known answers on small graphs (K4, two tets
  *                 sharing a face, a path), a literal sorted-set restatement in
- *                 Python on small meshes, proper colouring invariants.
- *   FEM consts    sum V = element volume, JX_inv J = I.
- *   masses        sum = density * volume for both distributions.
- *   materials     P = dW/dF and H = d2W/dx_r^2 by finite differences of the
+ *                 Python (tests/ref_literal.py) on small meshes, proper
+ *                 colouring invariants.
+ *   FEM consts    O: sum V = element volume, JX_inv J = I.
+ *   masses        O: sum = density * volume for both distributions; G.
+ *   materials     O: P = dW/dF and H = d2W/dx_r^2 by finite differences of the
  *                 energies; P_snh(I) = 0, P_nh(I) = mu I.
- *   VBD step      free fall (lambda = mu = 0) closed form; one free vertex
- *                 reduces to one Newton step of a convex problem checked by
- *                 brute force; energy decrease of the local solve.
- *   XPBD step     single-element closed forms of V0 and V2, V0 == V2 algebra
- *                 when E > 0.
- *   explicit      f = -dE/dx by finite differences of the total energy;
- *                 free-fall closed form.
+ *   VBD step      O: free fall (lambda = mu = 0) closed form; one free vertex
+ *                 = the brute-force Newton minimiser; Jacobi == Gauss-Seidel
+ *                 under a proper colouring; Chebyshev keeps free fall;
+ *                 residual vanishes at convergence.  P: the damping branch
+ *                 (the damped fixed point solves the damped local equation,
+ *                 derivatives by finite differences).
+ *   XPBD step     P: free-fall closed form of the semi-implicit frame, the
+ *                 global-damping velocity update in closed form, V0 on one
+ *                 P1 tet and V2 on one P2 tet against hand derivations with
+ *                 finite-difference energy gradients.  O: V0 == V2 when E > 0.
+ *   explicit      O: f = -dE/dx by finite differences of the total energy.
+ *                 P: free-fall closed form, f = -dE/dx - k H_vv v with H_vv
+ *                 by finite differences.
+ *   diagnostics   O: rest, uniaxial stretch and reflection closed forms.
+ * The restated reference CPU paths (cpu_xpbd_impl.h, cpu_vbd_impl.h) are
+ * pinned in P: free fall, a hand-traced two-element XPBD case (insertion
+ * order, accumulated multipliers, RB_Fixation, global damping), converged
+ * CPU VBD steps equal to this oracle's, float twins within float rounding.
  */
 #include "oracle.h"
 

@@ -4,9 +4,12 @@
  * This library is the checker: tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it to compare against the HIP engine.
  * Nothing in the product (octopusengine_amd/, liboe.so) links, loads or calls
- * it.  It restates the reference's algorithms in plain C, in double precision,
- * written independently of the product code (closed-form shape functions,
- * indexed-heap colourings, the reference's six-block Hessian assembly).
+ * it.  It restates the reference's algorithms in plain C, in double precision:
+ * closed-form shape functions (the product inverts a Vandermonde matrix), the
+ * reference's six-block Hessian assembly (the product uses a closed form), and
+ * colourings.  The colourings share the product's data-structure design (CSR
+ * multigraph, indexed heap); their independent pins are the literal
+ * sorted-set restatement in tests/ref_literal.py and hand-traced graphs.
  *
  * Each function cites the reference file:line it follows (paths relative to
  * octopus/ of ShortyBatou/OctopusEngine).  How each one is pinned is listed in
@@ -95,6 +98,26 @@ orc_cpu_xpbd_d* orc_cpu_xpbd_create_d(int element, int nv, const float* xyz, int
 void orc_cpu_xpbd_step_d(orc_cpu_xpbd_d* s, double dt);
 void orc_cpu_xpbd_positions_d(orc_cpu_xpbd_d* s, double* out);
 void orc_cpu_xpbd_destroy_d(orc_cpu_xpbd_d* s);
+
+/* Reference CPU VBD path (VertexBlockDescent + VBD_FEM + RB_Fixation,
+ * sequential Gauss-Seidel in vertex id order), in float (the timed CPU
+ * baseline) and in double.  Stable Neo-Hookean only (NULL otherwise). */
+typedef struct orc_cpu_vbd_f orc_cpu_vbd_f;
+typedef struct orc_cpu_vbd_d orc_cpu_vbd_d;
+orc_cpu_vbd_f* orc_cpu_vbd_create_f(int element, int nv, const float* xyz, int ne, const int* topo, int material,
+                                    double young, double poisson, double density, int femshape, int iterations,
+                                    int substeps, double damping, double rho, const int* fixed, int nfixed,
+                                    const double gravity[3]);
+void orc_cpu_vbd_step_f(orc_cpu_vbd_f* s, double dt);
+void orc_cpu_vbd_positions_f(orc_cpu_vbd_f* s, double* out);
+void orc_cpu_vbd_destroy_f(orc_cpu_vbd_f* s);
+orc_cpu_vbd_d* orc_cpu_vbd_create_d(int element, int nv, const float* xyz, int ne, const int* topo, int material,
+                                    double young, double poisson, double density, int femshape, int iterations,
+                                    int substeps, double damping, double rho, const int* fixed, int nfixed,
+                                    const double gravity[3]);
+void orc_cpu_vbd_step_d(orc_cpu_vbd_d* s, double dt);
+void orc_cpu_vbd_positions_d(orc_cpu_vbd_d* s, double* out);
+void orc_cpu_vbd_destroy_d(orc_cpu_vbd_d* s);
 
 #ifdef __cplusplus
 }

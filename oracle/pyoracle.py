@@ -74,6 +74,13 @@ for sfx in ("_f", "_d"):
     getattr(lib, "orc_cpu_xpbd_step" + sfx).argtypes = [P, C.c_double]
     getattr(lib, "orc_cpu_xpbd_positions" + sfx).argtypes = [P, f64p]
     getattr(lib, "orc_cpu_xpbd_destroy" + sfx).argtypes = [P]
+    cr = getattr(lib, "orc_cpu_vbd_create" + sfx)
+    cr.restype = P
+    cr.argtypes = [C.c_int, C.c_int, f32p, C.c_int, i32p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                   C.c_int, C.c_int, C.c_double, C.c_double, i32p, C.c_int, f64p]
+    getattr(lib, "orc_cpu_vbd_step" + sfx).argtypes = [P, C.c_double]
+    getattr(lib, "orc_cpu_vbd_positions" + sfx).argtypes = [P, f64p]
+    getattr(lib, "orc_cpu_vbd_destroy" + sfx).argtypes = [P]
 
 
 def _f32(a):
@@ -287,5 +294,37 @@ class CpuXPBD:
     def __del__(self):
         try:
             getattr(lib, "orc_cpu_xpbd_destroy" + self._sfx)(self._h)
+        except Exception:
+            pass
+
+
+class CpuVBD:
+    """Reference CPU VBD path: sequential Gauss-Seidel over the vertices in id
+    order (float = timed baseline, double = its FP64 twin)."""
+
+    def __init__(self, element, xyz, topo, fixed, *, material=STABLE_NEOHOOKE, young=5e5, poisson=0.49,
+                 density=1000.0, femshape=True, iterations=10, substeps=1, damping=0.0, rho=0.0,
+                 gravity=(0.0, -9.81, 0.0), precision="f"):
+        self._sfx = "_f" if precision == "f" else "_d"
+        self._xyz, self._topo, self._fix = _f32(xyz), _i32(topo), _i32(fixed)
+        self.N = self._xyz.shape[0]
+        create = getattr(lib, "orc_cpu_vbd_create" + self._sfx)
+        self._h = create(element, self.N, fp(self._xyz), self._topo.shape[0], ip(self._topo), material, young, poisson,
+                         density, int(femshape), iterations, substeps, damping, rho, ip(self._fix), self._fix.size,
+                         dp(_f64(gravity)))
+        if not self._h:
+            raise ValueError("CPU VBD supports Stable Neo-Hookean only")
+
+    def step(self, dt):
+        getattr(lib, "orc_cpu_vbd_step" + self._sfx)(self._h, dt)
+
+    def positions(self):
+        out = np.empty((self.N, 3))
+        getattr(lib, "orc_cpu_vbd_positions" + self._sfx)(self._h, dp(out))
+        return out
+
+    def __del__(self):
+        try:
+            getattr(lib, "orc_cpu_vbd_destroy" + self._sfx)(self._h)
         except Exception:
             pass
