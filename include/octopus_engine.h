@@ -147,8 +147,11 @@ oe_status oe_scene_get_timing(oe_scene* s, double* total_ms, int64_t* launches);
 /* Many VBD scenes stepped together: one launch per global colour index and
  * element type covers every scene of the batch (the reference steps scene after
  * scene, src/Core/Engine.cpp:15-21).  Each scene keeps its own results exactly
- * as if stepped alone.  The scenes must outlive the batch; a batch holds no
- * copy of their state. */
+ * as if stepped alone.  A batch holds no copy of the scenes' positions and
+ * velocities, but copies their fixed sets, box limits and task tables when it
+ * is created: while a scene belongs to a batch its constraint and tile changes
+ * return OE_ERR_STATE.  A scene appears once per batch and cannot be held by a
+ * server; destroying a scene makes its batches refuse to step (OE_ERR_STATE). */
 oe_status oe_batch_create(oe_engine* e, oe_scene* const* scenes, int32_t n, oe_batch** out);
 oe_status oe_batch_destroy(oe_batch* b);
 oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps);
@@ -175,12 +178,22 @@ oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_u
  * is 4, set in the environment before the process first uses HIP. */
 oe_status oe_server_create(oe_engine* e, int32_t max_batch, int32_t streams, oe_server** out);
 oe_status oe_server_destroy(oe_server* sv);
-/* Register a scene; returns its slot id.  The scene must outlive the server. */
+/* Register a scene; returns its slot id.  A scene joins at most one server,
+ * once, and not while it belongs to a batch (OE_ERR_INVALID / OE_ERR_STATE).
+ * While it is held: oe_scene_step, oe_scene_set_state and constraint or tile
+ * changes return OE_ERR_STATE; reads (positions, velocities, diagnostics,
+ * residual, VTK) first wait for every request submitted for the scene;
+ * oe_scene_destroy waits for them and releases the slot. */
 oe_status oe_server_add_scene(oe_server* sv, oe_scene* s, int32_t* slot);
-/* Thread-safe; records the arrival time of the request. */
+/* Thread-safe; records the arrival time of the request.  Requests of one
+ * scene run in submission order. */
 oe_status oe_server_submit(oe_server* sv, int32_t slot, float dt, int32_t substeps, uint64_t* ticket);
 /* Wait until every submitted request is complete. */
 oe_status oe_server_drain(oe_server* sv);
+/* Wait until the request with this ticket is complete (thread-safe). */
+oe_status oe_server_wait(oe_server* sv, uint64_t ticket);
+/* *done = 1 once the request with this ticket is complete, else 0 (thread-safe). */
+oe_status oe_server_poll(oe_server* sv, uint64_t ticket, int32_t* done);
 /* Since creation: completed requests, latency percentiles (ms, arrival to
  * completion), vertex-updates, batches and the mean batch size. */
 oe_status oe_server_stats(oe_server* sv, int64_t* completed, double* p50_ms, double* p99_ms, double* max_ms,

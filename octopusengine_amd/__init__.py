@@ -528,6 +528,15 @@ class Server:
     def drain(self):
         L.check(L.lib.oe_server_drain(self._h))
 
+    def wait(self, ticket: int):
+        """Block until the request with this ticket is complete."""
+        L.check(L.lib.oe_server_wait(self._h, ticket))
+
+    def poll(self, ticket: int) -> bool:
+        done = C.c_int32()
+        L.check(L.lib.oe_server_poll(self._h, ticket, C.byref(done)))
+        return bool(done.value)
+
     def stats(self):
         c, vu, nb = C.c_int64(), C.c_int64(), C.c_int64()
         p50, p99, mx, mb = C.c_double(), C.c_double(), C.c_double(), C.c_double()

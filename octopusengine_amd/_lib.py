@@ -112,6 +112,8 @@ _SIGS = {
     "oe_server_add_scene": (C.c_int32, [P, P, i32p]),
     "oe_server_submit": (C.c_int32, [P, C.c_int32, C.c_float, C.c_int32, u64p]),
     "oe_server_drain": (C.c_int32, [P]),
+    "oe_server_wait": (C.c_int32, [P, C.c_uint64]),
+    "oe_server_poll": (C.c_int32, [P, C.c_uint64, i32p]),
     "oe_server_stats": (C.c_int32, [P, i64p, f64p, f64p, f64p, i64p, i64p, f64p]),
     "oe_server_latencies": (C.c_int32, [P, f64p, C.c_int64, i64p]),
     "oe_host_beam_mesh": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,

@@ -302,18 +302,38 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
 
 Batch::Batch(Engine* e, std::vector<Scene*> sc) : eng(e), scenes(std::move(sc)), tiles(default_tiles()) {
     if (scenes.empty()) throw std::invalid_argument("empty batch");
-    for (Scene* s : scenes) {
+    for (size_t i = 0; i < scenes.size(); ++i) {
+        Scene* s = scenes[i];
         if (!s) throw std::invalid_argument("null scene in batch");
         if (s->eng != eng) throw std::invalid_argument("scene belongs to another engine");
+        // two segments of one launch would update the same vertices
+        if (std::find(scenes.begin(), scenes.begin() + static_cast<long>(i), s) != scenes.begin() + static_cast<long>(i))
+            throw std::invalid_argument("a scene appears twice in the batch");
+        if (s->server) throw std::logic_error("the scene is held by a server");
         check_batchable(*s);
     }
     flags.reserve(scenes.size());
     for (Scene* s : scenes) flags.emplace_back(&eng->pool, fixed_flags(*s));
     d_scenes.allocate(&eng->pool, scenes.size());
     rebuild();
+    for (Scene* s : scenes) s->batches.push_back(this);
 }
 
-Batch::~Batch() { drop_graph(); }
+Batch::~Batch() {
+    drop_graph();
+    for (Scene* s : scenes) {
+        if (!s) continue;
+        auto& v = s->batches;
+        v.erase(std::remove(v.begin(), v.end(), this), v.end());
+    }
+}
+
+void Batch::scene_destroyed(Scene* s) {
+    for (Scene*& p : scenes)
+        if (p == s) p = nullptr;
+    broken = true;
+    drop_graph();
+}
 
 void Batch::rebuild() {
     drop_graph();
@@ -327,6 +347,7 @@ void Batch::rebuild() {
 }
 
 void Batch::set_tile(Element e, GatherTile t) {
+    if (broken) throw std::logic_error("a scene of this batch was destroyed");
     tiles[e] = t;
     rebuild();
 }
@@ -344,6 +365,7 @@ void Batch::drop_graph() {
 }
 
 void Batch::step(float dt, int substeps) {
+    if (broken) throw std::logic_error("a scene of this batch was destroyed");
     if (!(dt > 0.f) || substeps <= 0) throw std::invalid_argument("step needs dt > 0 and substeps > 0");
     if (!use_graph || timer.enabled) {
         launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, timer.enabled ? &timer : nullptr);

@@ -97,6 +97,10 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
 
 std::map<int, GatherTile> default_tiles();
 
+// A batch registers itself with its scenes (Scene::batches).  A scene may be
+// in several batches but appear once per batch, and a scene held by a server
+// cannot join one.  Destroying a scene first detaches it: the batch then
+// refuses to step (std::logic_error).
 struct Batch {
     Batch(Engine* eng, std::vector<Scene*> scenes);
     ~Batch();
@@ -104,6 +108,7 @@ struct Batch {
     void set_graph(bool on);
     void set_tile(Element e, GatherTile t);
     void rebuild();
+    void scene_destroyed(Scene* s);
 
     Engine* eng;
     std::vector<Scene*> scenes;
@@ -115,6 +120,7 @@ struct Batch {
     GatherTimer timer;
 
 private:
+    bool broken = false;   // a scene of the batch was destroyed
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

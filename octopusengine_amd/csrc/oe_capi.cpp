@@ -92,6 +92,24 @@ oe::HostMesh mesh_from(int32_t element, const float* xyz, int32_t nv, const int3
 
 void copy3(std::vector<float>& src, float* dst) { std::memcpy(dst, src.data(), src.size() * sizeof(float)); }
 
+// Reads of a served scene first wait for every request submitted for it.
+void wait_served(oe::Scene& sc) {
+    if (sc.server) sc.server->wait_slot(sc.server_slot);
+}
+
+// A server steps its scenes on its own streams: nothing else may step them or
+// write their state.
+void require_not_served(const oe::Scene& sc, const char* what) {
+    if (sc.server) throw std::logic_error(std::string(what) + ": the scene is held by a server");
+}
+
+// Batches and servers copy a scene's constraints and task tables when they
+// take the scene in.
+void require_unshared(const oe::Scene& sc, const char* what) {
+    require_not_served(sc, what);
+    if (!sc.batches.empty()) throw std::logic_error(std::string(what) + ": the scene belongs to a batch");
+}
+
 }  // namespace
 
 extern "C" {
@@ -192,12 +210,23 @@ oe_status oe_scene_create(oe_engine* e, const oe_scene_desc* d, oe_scene** out) 
 }
 
 oe_status oe_scene_destroy(oe_scene* s) {
-    return guard([&] { delete s; });
+    return guard([&] {
+        if (!s) return;
+        oe::Scene& sc = *s->scene;
+        // a served scene leaves its server once its requests are done; its
+        // batches refuse to step from now on
+        if (sc.server) sc.server->remove_scene(sc.server_slot);
+        const std::vector<oe::Batch*> batches = sc.batches;
+        for (oe::Batch* b : batches) b->scene_destroyed(&sc);
+        sc.batches.clear();
+        delete s;
+    });
 }
 
 oe_status oe_scene_fix_vertices(oe_scene* s, const int32_t* ids, int32_t n) {
     return guard([&] {
         OE_REQUIRE(s && (ids || n == 0) && n >= 0, "bad argument");
+        require_unshared(*s->scene, "fix_vertices");
         s->scene->fix_vertices(std::vector<int>(ids, ids + n));
     });
 }
@@ -205,6 +234,7 @@ oe_status oe_scene_fix_vertices(oe_scene* s, const int32_t* ids, int32_t n) {
 oe_status oe_scene_fix_plane(oe_scene* s, const float o[3], const float n[3], int32_t* n_fixed) {
     return guard([&] {
         OE_REQUIRE(s && o && n, "bad argument");
+        require_unshared(*s->scene, "fix_plane");
         const std::vector<int> ids = s->scene->select_plane(o, n);
         s->scene->fix_vertices(ids);
         if (n_fixed) *n_fixed = static_cast<int32_t>(ids.size());
@@ -214,6 +244,7 @@ oe_status oe_scene_fix_plane(oe_scene* s, const float o[3], const float n[3], in
 oe_status oe_scene_box_limit(oe_scene* s, const float pmin[3], const float pmax[3]) {
     return guard([&] {
         OE_REQUIRE(s && pmin && pmax, "bad argument");
+        require_unshared(*s->scene, "box_limit");
         s->scene->box_limit(pmin, pmax);
     });
 }
@@ -221,6 +252,7 @@ oe_status oe_scene_box_limit(oe_scene* s, const float pmin[3], const float pmax[
 oe_status oe_scene_step(oe_scene* s, float dt, int32_t substeps) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
+        require_not_served(*s->scene, "step");
         s->scene->step(dt, substeps);
     });
 }
@@ -249,6 +281,7 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
     return guard([&] {
         OE_REQUIRE(s, "null scene");
         oe::Scene& sc = *s->scene;
+        require_unshared(sc, "set_tile");
         check_tile(sc.mesh.element, lanes, qpl, mode);
         OE_REQUIRE(mode == oe::PASS_GATHER || sc.vbd_fem, "cached gradients and element tasks are VBD options");
         OE_REQUIRE(mode != oe::PASS_TASKS || sc.mesh.element != oe::Hexa27 || sc.vbd_fem->has_q2_basis,
@@ -313,6 +346,7 @@ oe_status oe_scene_info(oe_scene* s, int32_t* nv, int32_t* ne, int32_t* S, int32
     oe_status name(oe_scene* s, float* xyz) {                \
         return guard([&] {                                   \
             OE_REQUIRE(s && xyz, "bad argument");            \
+            wait_served(*s->scene);                          \
             std::vector<float> h;                            \
             s->scene->ps->method(h);                         \
             copy3(h, xyz);                                   \
@@ -328,6 +362,7 @@ OE_GETTER3(oe_scene_get_forces, get_forces)
 oe_status oe_scene_get_masses(oe_scene* s, float* m) {
     return guard([&] {
         OE_REQUIRE(s && m, "bad argument");
+        wait_served(*s->scene);
         std::vector<float> h;
         s->scene->ps->get_mass(h);
         std::memcpy(m, h.data(), h.size() * sizeof(float));
@@ -337,6 +372,7 @@ oe_status oe_scene_get_masses(oe_scene* s, float* m) {
 oe_status oe_scene_get_masks(oe_scene* s, int32_t* mask) {
     return guard([&] {
         OE_REQUIRE(s && mask, "bad argument");
+        wait_served(*s->scene);
         std::vector<int> h;
         s->scene->ps->get_masks(h);
         std::memcpy(mask, h.data(), h.size() * sizeof(int32_t));
@@ -358,6 +394,7 @@ oe_status oe_scene_get_colors(oe_scene* s, int32_t* colors) {
 oe_status oe_scene_set_state(oe_scene* s, const float* p, const float* last_p, const float* v) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
+        require_not_served(*s->scene, "set_state");
         s->scene->ps->set_state(p, last_p, v);
     });
 }
@@ -503,6 +540,20 @@ oe_status oe_server_drain(oe_server* sv) {
     });
 }
 
+oe_status oe_server_wait(oe_server* sv, uint64_t ticket) {
+    return guard([&] {
+        OE_REQUIRE(sv, "null server");
+        sv->server->wait(ticket);
+    });
+}
+
+oe_status oe_server_poll(oe_server* sv, uint64_t ticket, int32_t* done) {
+    return guard([&] {
+        OE_REQUIRE(sv && done, "bad argument");
+        *done = sv->server->poll(ticket) ? 1 : 0;
+    });
+}
+
 oe_status oe_server_stats(oe_server* sv, int64_t* completed, double* p50, double* p99, double* mx, int64_t* vu,
                           int64_t* batches, double* mean_batch) {
     return guard([&] {
@@ -621,6 +672,7 @@ oe_status oe_scene_element_diagnostics(oe_scene* s, float* stress, float* volume
                                        float* inverted) {
     return guard([&] {
         OE_REQUIRE(s, "null scene");
+        wait_served(*s->scene);
         s->scene->fem->diagnostics(s->scene->ps.get(), stress, volume, volume_diff, inverted);
     });
 }
@@ -629,6 +681,7 @@ oe_status oe_scene_residual(oe_scene* s, float dt, float* r) {
     return guard([&] {
         OE_REQUIRE(s && r, "bad argument");
         OE_REQUIRE(s->scene->vbd_fem, "the residual is defined for VBD scenes");
+        wait_served(*s->scene);
         std::vector<float> h;
         s->scene->vbd_fem->residual(s->scene->ps.get(), dt, h);
         std::memcpy(r, h.data(), h.size() * sizeof(float));
@@ -740,6 +793,7 @@ oe_status oe_scene_save_vtk(oe_scene* s, const char* path) {
     return guard([&] {
         OE_REQUIRE(s && path, "bad argument");
         oe::Scene& sc = *s->scene;
+        wait_served(sc);
         std::vector<float> rest, pos, m;
         sc.ps->get_init_position(rest);
         sc.ps->get_position(pos);

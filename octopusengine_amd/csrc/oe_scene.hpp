@@ -18,6 +18,9 @@
 
 namespace oe {
 
+class Server;
+struct Batch;
+
 enum Solver : int { SolverVBD = 0, SolverXPBD = 1, SolverExplicit = 2 };
 
 struct SceneConfig {
@@ -60,6 +63,15 @@ struct Scene {
     GPU_FEM_Explicit* explicit_fem = nullptr;
     GPU_Fix_Constraint* fix = nullptr;
     int64_t steps_done = 0;
+    // Host-side links to the batches and the server that hold this scene.
+    // A server steps the scene on its own streams, so while `server` is set
+    // direct steps, state writes and constraint changes are refused and reads
+    // first wait for the scene's submitted requests.  Batches copy the
+    // scene's constraints into their device tables when they are built, so
+    // constraint and tile changes are refused while `batches` is not empty.
+    Server* server = nullptr;
+    int server_slot = -1;
+    std::vector<Batch*> batches;
 
 private:
     void run_substeps(float dt, int substeps);

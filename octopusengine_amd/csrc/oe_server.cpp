@@ -30,21 +30,67 @@ Server::~Server() {
         }
         if (s.done) (void)hipEventDestroy(s.done);
     }
+    // requests still pending are dropped; the scenes are free again
+    for (Scene* s : scenes_)
+        if (s) {
+            s->server = nullptr;
+            s->server_slot = -1;
+        }
 }
 
 int Server::add_scene(Scene* s) {
     if (!s || s->eng != eng_) throw std::invalid_argument("scene belongs to another engine");
+    // one slot per scene: two slots would let two streams step the same buffers
+    if (s->server == this) throw std::invalid_argument("the scene is already held by this server");
+    if (s->server) throw std::logic_error("the scene is held by another server");
+    if (!s->batches.empty()) throw std::logic_error("the scene belongs to a batch");
     check_batchable(*s);
     std::unique_lock<std::mutex> lk(mu_);
     if (!error_.empty()) throw std::runtime_error(error_);
     // the device table is reallocated only while nothing is in flight
     idle_cv_.wait(lk, [&] { return (inflight_.empty() && launching_ == 0) || !error_.empty(); });
+    if (!error_.empty()) throw std::runtime_error(error_);
     flags_.emplace_back(&eng_->pool, fixed_flags(*s));
     host_table_.push_back(make_scene_dev(*s, flags_.back().get(), tiles_));
     scenes_.push_back(s);
     scene_busy_.push_back(0);
+    slot_open_.push_back(0);
     grow_table();
-    return static_cast<int>(scenes_.size()) - 1;
+    const int slot = static_cast<int>(scenes_.size()) - 1;
+    s->server = this;
+    s->server_slot = slot;
+    return slot;
+}
+
+void Server::wait(uint64_t ticket) {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (ticket == 0 || ticket >= next_ticket_) throw std::invalid_argument("unknown ticket");
+    done_cv_.wait(lk, [&] { return !open_tickets_.count(ticket) || !error_.empty(); });
+    if (!error_.empty()) throw std::runtime_error(error_);
+}
+
+bool Server::poll(uint64_t ticket) {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (ticket == 0 || ticket >= next_ticket_) throw std::invalid_argument("unknown ticket");
+    if (!error_.empty()) throw std::runtime_error(error_);
+    return !open_tickets_.count(ticket);
+}
+
+void Server::wait_slot(int slot) {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (slot < 0 || slot >= static_cast<int>(scenes_.size()) || !scenes_[slot])
+        throw std::invalid_argument("unknown scene slot");
+    done_cv_.wait(lk, [&] { return slot_open_[slot] == 0 || !error_.empty(); });
+    if (!error_.empty()) throw std::runtime_error(error_);
+}
+
+void Server::remove_scene(int slot) {
+    wait_slot(slot);
+    std::lock_guard<std::mutex> lk(mu_);
+    Scene* s = scenes_[slot];
+    scenes_[slot] = nullptr;   // submit() refuses the slot from now on
+    s->server = nullptr;
+    s->server_slot = -1;
 }
 
 void Server::grow_table() {
@@ -65,9 +111,12 @@ uint64_t Server::submit(int slot, float dt, int substeps) {
     {
         std::lock_guard<std::mutex> lk(mu_);
         if (!error_.empty()) throw std::runtime_error(error_);
-        if (slot < 0 || slot >= static_cast<int>(scenes_.size())) throw std::invalid_argument("unknown scene slot");
+        if (slot < 0 || slot >= static_cast<int>(scenes_.size()) || !scenes_[slot])
+            throw std::invalid_argument("unknown scene slot");
         t = next_ticket_++;
         pending_.push_back(Request{t, slot, dt, substeps, Clock::now()});
+        open_tickets_.insert(t);
+        slot_open_[slot]++;
         submitted_++;
     }
     cv_.notify_one();
@@ -100,6 +149,8 @@ void Server::retire(bool) {
             for (const Request& r : t.reqs) {
                 lat_ms_.push_back(std::chrono::duration<double, std::milli>(now - r.arrival).count());
                 scene_busy_[r.slot] = 0;
+                slot_open_[r.slot]--;
+                open_tickets_.erase(r.ticket);
             }
             completed_ += static_cast<int64_t>(t.reqs.size());
             vertex_updates_ += t.vertex_updates;
@@ -107,6 +158,7 @@ void Server::retire(bool) {
         }
         if (pending_.empty() && inflight_.empty() && launching_ == 0) idle_cv_.notify_all();
     }
+    done_cv_.notify_all();
 }
 
 bool Server::try_launch(std::unique_lock<std::mutex>& lk, int si) {
@@ -135,6 +187,8 @@ bool Server::try_launch(std::unique_lock<std::mutex>& lk, int si) {
             picked[r.slot] = 1;
             tick.reqs.push_back(r);
         } else {
+            // the scene's later requests wait behind this one
+            picked[r.slot] = 1;
             rest.push_back(r);
         }
     }
@@ -190,6 +244,7 @@ void Server::worker(int si) {
         if (error_.empty()) error_ = e.what();
         idle_cv_.notify_all();
         cv_.notify_all();
+        done_cv_.notify_all();
     }
 }
 

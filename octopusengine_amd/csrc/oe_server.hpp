@@ -16,6 +16,11 @@
 //     streams runs in parallel.
 // Scene state never leaves HBM; scenes are admitted against the engine's
 // memory budget when they are created.
+// Requests of one scene run in submission order: once a scene's oldest
+// pending request is passed over while a tick is formed, none of its later
+// requests joins that tick.  Every request gets a ticket; wait(ticket) and
+// poll(ticket) report its completion, wait_slot() that of all requests
+// submitted for one scene so far (scene reads use it).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,6 +31,7 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "oe_batch.hpp"
@@ -42,6 +48,10 @@ public:
     int add_scene(Scene* s);                                 // returns the slot id
     uint64_t submit(int slot, float dt, int substeps);       // thread-safe
     void drain();                                            // wait for every submitted request
+    void wait(uint64_t ticket);                              // wait for one request
+    bool poll(uint64_t ticket);                              // true once the request is complete
+    void wait_slot(int slot);                                // wait for every request of one scene
+    void remove_scene(int slot);                             // wait_slot, then release the scene
     struct Stats {
         int64_t completed = 0, vertex_updates = 0, batches = 0, max_batch_seen = 0;
         double p50_ms = 0, p99_ms = 0, max_ms = 0, mean_batch = 0;
@@ -87,6 +97,9 @@ private:
     std::map<int, GatherTile> tiles_;
     std::vector<StreamSlot> slots_;
     std::vector<char> scene_busy_;
+    std::vector<int64_t> slot_open_;           // submitted and not yet completed, per slot
+    std::unordered_set<uint64_t> open_tickets_;
+    std::condition_variable done_cv_;          // a request completed (or an error)
     std::deque<Request> pending_;
     std::vector<Tick> inflight_;
     int launching_ = 0;   // ticks taken from pending_ and not yet in inflight_
