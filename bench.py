@@ -61,6 +61,11 @@ def algorithmic_bytes_per_iteration(n_vertices: int, n_elements: int, S: int, Q:
     return n_elements * S * (40.0 * Q + 16.0 * S + 8.0) + 56.0 * n_vertices
 
 
+def progress(msg: str) -> None:
+    """One line per finished leg on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_info() -> tuple[str, int]:
     model = platform.processor() or "unknown"
     try:
@@ -125,8 +130,25 @@ def measured_traffic(kernel_prefix: str, workload: str):
             return {"bytes_per_launch": r["read_bytes_per_launch"] + r["write_bytes_per_launch"],
                     "read_bytes_per_launch": r["read_bytes_per_launch"],
                     "write_bytes_per_launch": r["write_bytes_per_launch"],
-                    "dispatches": r["dispatches"], "source": f"profiles/traffic.json ({rec.get('commit', '?')})"}
+                    "dispatches": r["dispatches"], "source": "profiles/traffic.json"}
     return None
+
+
+def kernel_roofline(name: str, alg_bytes_per_launch: float, avg_launch_s: float, launches: int, traffic) -> dict:
+    """Roofline of one colour-pass kernel: `achieved` = the algorithmic bytes
+    of the hot-path model per launch / the kernel's event-timed average launch
+    duration; `traffic` = HBM bytes it really moves per launch (rocprof), and
+    `traffic_frac` = that traffic / duration / peak."""
+    achieved = alg_bytes_per_launch / max(avg_launch_s, 1e-12) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes_per_launch"] if traffic else None,
+           "kernel": name, "bytes_per_launch": alg_bytes_per_launch, "avg_launch_us": avg_launch_s * 1e6,
+           "launches": launches}
+    if traffic:
+        out["traffic_gbs"] = traffic["bytes_per_launch"] / max(avg_launch_s, 1e-12) / 1e9
+        out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
+        out["traffic_detail"] = traffic
+    return out
 
 
 def batched_leg(args, eng) -> dict:
@@ -143,13 +165,35 @@ def batched_leg(args, eng) -> dict:
     vu_step = info["vertex_updates"]
     for _ in range(args.warmup):
         b.step(DT, 1)
+    # the two smallest scenes (one per element type) after the warm-up steps,
+    # for the accuracy check against the oracle
+    early = {"steps": args.warmup, "specs": [], "positions": []}
+    for el in (oe.TETRA10, oe.HEXA27):
+        idx = [i for i, sp in enumerate(specs) if sp[0] == el]
+        if idx:
+            i = min(idx, key=lambda k: scenes[k].n_vertices)
+            early["specs"].append(specs[i])
+            early["positions"].append(scenes[i].positions())
     # timed region (value): eager launches, as Batch.step issues them
     wall = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
     # roofline: as many steps again with HIP events around every colour-pass
     # launch, on the stream the kernels run on (kernel durations only)
     b.set_timing(True)
     b.get_timing()
+    kinds = [(oe.TETRA10, "k_batch_fused_tasks<10,4> (P2 colour pass)", "k_batch_fused_tasks<10, 4, false>"),
+             (oe.HEXA27, "k_batch_q2_fused (Q2 colour pass)", "void k_batch_q2_fused<false>")]
+    for el, _, _ in kinds:
+        b.get_timing_element(el)
     wall_events = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
+    per_kernel = {}
+    for el, name, prefix in kinds:
+        ms, n = b.get_timing_element(el)
+        ie = b.info_element(el)
+        if n == 0 or ie["gather_launches"] == 0:
+            continue
+        per_kernel[el] = kernel_roofline(name, ie["gather_bytes"] / ie["gather_launches"], ms / 1e3 / n, n,
+                                         measured_traffic(prefix, f"batched{args.scenes}_seed{args.seed}"))
+        per_kernel[el]["ms_per_step"] = ms / args.steps
     gather_ms, launches = b.get_timing()
     b.set_timing(False)
     wall_graph, wall_plain, lat = wall, float("nan"), [float("nan")]
@@ -174,12 +218,13 @@ def batched_leg(args, eng) -> dict:
 
         plain_step()
         wall_plain = timed(plain_step, plain_steps, eng.synchronize)
-    finite = all(bool(np.isfinite(s.positions()).all()) for s in scenes[:8])
+    finite = all(bool(np.isfinite(s.positions()).all()) for s in scenes)
     bytes_total = info["gather_bytes"] * args.steps
     avg_launch_s = gather_ms / 1e3 / max(launches, 1)
     achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
-    traffic = measured_traffic("vbd_colour_pass", f"batched{args.scenes}_seed{args.seed}")
     b.close()
+    # the kernel with the largest share of the step is the headline roofline
+    dominant = max(per_kernel.values(), key=lambda r: r["ms_per_step"])
     return {
         "value": vu_step * args.steps / wall,
         "graph_value": vu_step * args.steps / wall_graph,
@@ -190,18 +235,16 @@ def batched_leg(args, eng) -> dict:
         "p99_substep_ms": float(np.percentile(lat, 99)),
         "plain_value": vu_step * plain_steps / wall_plain,
         "plain_ms_per_step": wall_plain / plain_steps * 1e3,
-        "speedup_vs_plain": (vu_step * args.steps / wall_graph) / (vu_step * plain_steps / wall_plain),
+        # both eager: the headline timed region against the plain loop's eager launches
+        "speedup_vs_plain": (vu_step * args.steps / wall) / (vu_step * plain_steps / wall_plain),
         "scenes": len(scenes), "p2_scenes": n_p2, "vertices": nv, "vertex_updates_per_step": vu_step,
         "gather_launches_per_step": info["gather_launches"], "setup_s": setup_s, "positions_finite": finite,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "traffic_detail": traffic,
-                     "kernel": "VBD colour pass over all scenes: k_batch_fused_tasks<10,4> (P2), "
-                               "k_batch_q2_fused (Q2)",
-                     "bytes_per_launch": bytes_total / max(launches, 1), "avg_launch_us": avg_launch_s * 1e6,
-                     "launches": launches},
+        "roofline": dominant,
+        "roofline_by_kernel": list(per_kernel.values()),
+        "colour_pass_model_frac_all_kernels": achieved / HBM_PEAK_GBS,
+        "colour_pass_avg_launch_us_all_kernels": avg_launch_s * 1e6,
         "_scenes": scenes,
+        "_early": early,
     }
 
 
@@ -248,24 +291,97 @@ def plain_single_leg(args, eng) -> dict:
 
 
 def cpu_baseline_leg(steps: int) -> dict:
-    """FP64 oracle VBD (single thread) on a bounded sample of the single-scene workload."""
+    """The reference's CPU VBD path (VertexBlockDescent + VBD_FEM: sequential
+    Gauss-Seidel over the vertices in id order, float, one thread), restated
+    in C, on a bounded sample of the single-scene workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import octopusengine_amd as oe
     import pyoracle as O
 
     xyz, topo = oe.beam_mesh(oe.TETRA10, (64, 16, 16), SIZE)
-    o = O.Scene(O.TETRA10, xyz, topo, material=O.STABLE_NEOHOOKE, young=5e5, poisson=0.49, iterations=ITERS)
+    c = O.CpuVBD(O.TETRA10, xyz, topo, np.nonzero(xyz[:, 0] < 1e-6)[0], material=O.STABLE_NEOHOOKE, young=5e5,
+                 poisson=0.49, iterations=ITERS)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c.step(DT)
+    dt = time.perf_counter() - t0
+    model, ncpu = cpu_info()
+    return {"value": xyz.shape[0] * ITERS * steps / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps x {ITERS} iterations of the P2 64x16x16 beam (140481 vertices) on the "
+                      f"reference's CPU VBD path restated in C (float, sequential Gauss-Seidel), 1 thread, host "
+                      f"'{model}' ({ncpu} cpus visible)",
+            "seconds": dt}
+
+
+def _oracle_run(spec, steps):
+    """FP64 oracle positions of a beam scene (element, cells, E, nu) after
+    `steps` VBD steps (fixed x = 0 face, box limit, dt = 0.005, 10 iterations)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import octopusengine_amd as oe
+    import pyoracle as O
+
+    el, cells, E, nu = spec
+    xyz, topo = oe.beam_mesh(el, cells, SIZE)
+    o = O.Scene(el, xyz, topo, material=O.STABLE_NEOHOOKE, young=E, poisson=nu, iterations=ITERS)
     o.fix(np.nonzero(xyz[:, 0] < 1e-6)[0])
     o.box((-10, -5, -10), (10, 5, 10))
     t0 = time.perf_counter()
     for _ in range(steps):
         o.step(DT, 1)
-    dt = time.perf_counter() - t0
-    model, ncpu = cpu_info()
-    return {"value": xyz.shape[0] * ITERS * steps / dt, "unit": "vertex-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps x {ITERS} iterations of the P2 64x16x16 beam (140481 vertices) with the "
-                      f"FP64 oracle, 1 thread, host '{model}' ({ncpu} cpus visible)",
-            "seconds": dt}
+    return o.positions(), time.perf_counter() - t0
+
+
+def accuracy_leg(args, early) -> dict:
+    """Max vertex-position error of GPU results against FP64 oracle runs of
+    the same scenes, as a fraction of the bounding-box diagonal (targets:
+    1e-5 after 1 step, 1e-3 after 100):
+      * the two smallest scenes (one P2, one Q2) of the batched workload,
+        read from the timed batch after its warm-up steps;
+      * the same two scenes stepped 100 times in a batch of their own (the
+        batched kernels);
+      * the single-scene P2 64x16x16 beam after `cpu_steps` steps.
+    The oracle runs go to a thread pool (the C oracle releases the GIL)."""
+    import octopusengine_amd as oe
+
+    diag = float(np.sqrt(sum(v * v for v in SIZE)))
+    long_steps = 100
+    eng = oe.Engine(0)
+    small = early["specs"]
+    twins = [oe.make_scene(eng, el, cells, SIZE, young=E, poisson=nu, iterations=ITERS) for el, cells, E, nu in small]
+    tb = oe.Batch(eng, twins)
+    for _ in range(long_steps):
+        tb.step(DT, 1)
+    long_gpu = [t.positions() for t in twins]
+    tb.close()
+    for t in twins:
+        t.close()
+    single = oe.make_scene(eng, oe.TETRA10, (64, 16, 16), SIZE, young=5e5, poisson=0.49, iterations=ITERS)
+    for _ in range(args.cpu_steps):
+        single.step(DT, 1)
+    single_gpu = single.positions()
+    single.close()
+    eng.close()
+    jobs = [(sp, early["steps"]) for sp in small] + [(sp, long_steps) for sp in small] + \
+           [((oe.TETRA10, (64, 16, 16), 5e5, 0.49), args.cpu_steps)]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        res = list(ex.map(lambda j: _oracle_run(*j), jobs))
+    out = []
+    names = ["P2", "Q2"]
+    for i, sp in enumerate(small):
+        el, cells, E, nu = sp
+        tag = f"{names[i]} {cells[0]}x{cells[1]}x{cells[2]} (E={E:.3g}, nu={nu:.3f})"
+        e1 = float(np.abs(early["positions"][i].astype(np.float64) - res[i][0]).max())
+        e2 = float(np.abs(long_gpu[i].astype(np.float64) - res[len(small) + i][0]).max())
+        out.append({"scene": tag + ", the timed 256-scene batch", "steps": early["steps"], "max_err_m": e1,
+                    "frac_of_diag": e1 / diag})
+        out.append({"scene": tag + ", batch of the two smallest scenes", "steps": long_steps, "max_err_m": e2,
+                    "frac_of_diag": e2 / diag})
+    e3 = float(np.abs(single_gpu.astype(np.float64) - res[-1][0]).max())
+    out.append({"scene": "P2 64x16x16 single scene (E=5e5, nu=0.49)", "steps": args.cpu_steps, "max_err_m": e3,
+                "frac_of_diag": e3 / diag})
+    return {"vs": "FP64 oracle of the same scene (oracle/oracle.c)", "diag_m": diag, "results": out,
+            "max_frac_of_diag_after_100_steps": max(r["frac_of_diag"] for r in out if r["steps"] >= 100),
+            "oracle_seconds": sum(r[1] for r in res)}
 
 
 def cpu_xpbd_reference_leg() -> dict:
@@ -409,7 +525,7 @@ def main():
     ap.add_argument("--no-single", action="store_true", help="skip the single-scene leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--serve-pool", type=int, default=1024, help="resident scenes in the served pool")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson request stream")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the Poisson request stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="step requests per second")
     ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -427,8 +543,11 @@ def main():
 
     eng = oe.Engine(0)
     bat = batched_leg(args, eng)
+    progress(f"batched leg: {bat['ms_per_step']:.2f} ms per step")
     scenes = bat.pop("_scenes")
+    early = bat.pop("_early")
     roof = bat.pop("roofline")
+    roof_by_kernel = bat.pop("roofline_by_kernel")
     out = {
         "metric": "VBD vertex-updates/s, batched (256 mixed P2/Q2 scenes) vs plain per-scene loop",
         "value": bat["value"],
@@ -448,15 +567,18 @@ def main():
                    "vertices": bat["vertices"], "vertex_updates_per_step": bat["vertex_updates_per_step"],
                    "gather_launches_per_step": bat["gather_launches_per_step"],
                    "launch": "eager (value); roofline from a second run of as many steps with HIP events "
-                             "around every colour-pass launch"},
+                             "around every colour-pass launch, for the kernel with the largest share of a step "
+                             "(every colour-pass kernel in roofline_by_kernel)"},
         "roofline": roof,
+        "roofline_by_kernel": roof_by_kernel,
         "p50_substep_ms": bat["p50_substep_ms"],
         "p99_substep_ms": bat["p99_substep_ms"],
         "latency_source": "batched step of all 256 scenes (graph replay)",
         "batched": {k: bat[k] for k in ("graph_value", "graph_ms_per_step", "events_ms_per_step", "plain_value",
                                         "plain_ms_per_step",
                                         "speedup_vs_plain", "p50_substep_ms", "p99_substep_ms", "setup_s",
-                                        "positions_finite")},
+                                        "positions_finite", "colour_pass_model_frac_all_kernels",
+                                        "colour_pass_avg_launch_us_all_kernels")},
     }
     for sc in scenes:
         sc.close()
@@ -467,6 +589,7 @@ def main():
     eng.close()
     if not args.no_serve:
         out["served"] = served_leg(args)
+        progress(f"served leg: p50 {out['served']['p50_ms']:.2f} ms, p99 {out['served']['p99_ms']:.2f} ms")
         # the substep-latency target is a serving figure: request (one step)
         # latency under the Poisson load
         out["p50_substep_ms"] = out["served"]["p50_ms"]
@@ -475,12 +598,17 @@ def main():
     if not args.no_single:
         eng = oe.Engine(0)
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
+        progress(f"single-scene leg: {out['plain_single_scene_p2_64x16x16']['ms_per_step']:.3f} ms per step")
         eng.close()
     if args.copy_bytes > 0:
         out["copy_calibration"] = copy_calibration(args.copy_bytes)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_leg(args.cpu_steps)
+        progress(f"cpu baseline: {out['cpu_baseline']['value']:.3e} vertex-updates/s")
         out["cpu_reference_xpbd_p1_16x4x4"] = cpu_xpbd_reference_leg()
+        if early["steps"] > 0:
+            out["accuracy"] = accuracy_leg(args, early)
+            progress(f"accuracy leg: max {out['accuracy']['max_frac_of_diag_after_100_steps']:.2e} of the diagonal")
     print(json.dumps(_finite_or_null(out)), flush=True)
 
 

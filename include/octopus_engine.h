@@ -118,6 +118,44 @@ oe_status oe_scene_set_graph(oe_scene* s, int32_t enable);
  * (src/GPU/VBD/GPU_VBD_FEM.cu:712-756). */
 oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mode);
 oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* mode);
+/* The full colour-pass layout: the gather knobs above plus, for element
+ * tasks, the incidences per block (the elements one workgroup stages in LDS;
+ * 0 = the kernel's maximum, 256 for tetrahedra and 8-node hexahedra, 32 for
+ * 27-node hexahedra) and the throughput-form block count below which a
+ * colour runs in the kernels' latency form (default 512).  No counterpart in
+ * the reference (src/GPU/VBD/GPU_VBD_FEM.cu:619-650 fixes one layout per
+ * kernel version). */
+typedef struct oe_tile {
+    int32_t lanes;
+    int32_t qpl;
+    int32_t mode;
+    int32_t inc_per_block;
+    int64_t latency_blocks;
+} oe_tile;
+oe_status oe_scene_set_tile_ex(oe_scene* s, const oe_tile* t);
+oe_status oe_scene_get_tile_ex(oe_scene* s, oe_tile* t);
+
+/* Tile autotuner (no counterpart in the reference).  Times every layout the
+ * kernels accept for the scene's element type on the device (one colour sweep
+ * each, with HIP events on the engine's stream; the state the sweeps move is
+ * saved and put back), applies the fastest and caches it per (device,
+ * element type, floor(log2(vertices)), scene or batch) for the rest of the
+ * process.  A cached layout is applied without timing unless force != 0.
+ * VBD and explicit scenes; not while the scene belongs to a batch or a server
+ * (OE_ERR_STATE).  Outputs may be NULL: the chosen layout, its time per sweep
+ * (ms) and the number of layouts timed (0 = taken from the cache). */
+oe_status oe_scene_autotune(oe_scene* s, int32_t force, oe_tile* chosen, double* sweep_ms, int32_t* candidates);
+/* When enabled, oe_scene_create tunes every VBD / explicit scene it creates on
+ * this engine (cached layouts apply without timing). */
+oe_status oe_engine_set_autotune(oe_engine* e, int32_t enable);
+/* Entries of the autotuner's cache (all devices): capacity in *n, count out;
+ * per entry device, element, size bucket, form (0 scene, 1 batch), layout,
+ * time (ms per sweep, or per batched step of that element type's colour
+ * passes) and candidates timed.  Any array may be NULL. */
+oe_status oe_autotune_cache(int32_t* n, int32_t* device, int32_t* element, int32_t* bucket, int32_t* form,
+                            oe_tile* tiles, double* ms, int32_t* candidates);
+oe_status oe_autotune_clear(void);
+
 /* Element record layout chosen at creation: affine = 1 when every element is
  * an affine image of the reference element and its J^-1, |det J| are stored
  * once per element; 0 for one record per (element, quadrature point) as in
@@ -157,10 +195,25 @@ oe_status oe_batch_destroy(oe_batch* b);
 oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps);
 oe_status oe_batch_set_graph(oe_batch* b, int32_t enable);
 oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t mode);
+/* Full layout per element type (latency_blocks has no effect: batches run the
+ * throughput form); get returns the layout in use. */
+oe_status oe_batch_set_tile_ex(oe_batch* b, int32_t element, const oe_tile* t);
+oe_status oe_batch_get_tile_ex(oe_batch* b, int32_t element, oe_tile* t);
+/* Autotune every element type of the batch: the candidates are timed on the
+ * batch itself (the colour-pass launches of that type in one batched step,
+ * HIP events); the scenes' state is saved and put back.  Cached per (device,
+ * element type, floor(log2(vertices of that type)), batch) as above. */
+oe_status oe_batch_autotune(oe_batch* b, int32_t force);
 oe_status oe_batch_set_timing(oe_batch* b, int32_t enable);
 oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches);
 /* Gather launches and vertex-updates per substep; algorithmic gather bytes per substep. */
 oe_status oe_batch_info(oe_batch* b, int32_t* gather_launches, int64_t* vertex_updates, double* gather_bytes);
+/* The same two, restricted to the colour-pass launches of one element type
+ * (their own kernels): event-timed milliseconds and launch count since the
+ * last read for that type, and its share of the per-substep figures. */
+oe_status oe_batch_get_timing_element(oe_batch* b, int32_t element, double* gather_ms, int64_t* launches);
+oe_status oe_batch_info_element(oe_batch* b, int32_t element, int32_t* gather_launches, int64_t* vertex_updates,
+                                double* gather_bytes);
 
 /* ------------------------------------------------- serving (continuous batching) */
 /* A request loop over HBM-resident VBD scenes (no counterpart in the

@@ -275,6 +275,10 @@ class Engine:
     def handle(self):
         return self._h
 
+    def set_autotune(self, enable: bool):
+        """Tune the layout of every VBD / explicit scene created on this engine."""
+        L.check(L.lib.oe_engine_set_autotune(self._h, int(enable)))
+
     def synchronize(self):
         L.check(L.lib.oe_engine_synchronize(self._h))
 
@@ -363,6 +367,24 @@ class Scene:
         and mode (PASS_GATHER; PASS_CACHE: cached gradients with qpl = 1;
         PASS_TASKS: element tasks, which ignore lanes and qpl)."""
         L.check(L.lib.oe_scene_set_tile(self._h, lanes, qpl, int(mode)))
+
+    def set_tile_ex(self, lanes=64, qpl=1, mode=PASS_GATHER, inc_per_block=0, latency_blocks=512):
+        t = L.Tile(lanes, qpl, int(mode), inc_per_block, latency_blocks)
+        L.check(L.lib.oe_scene_set_tile_ex(self._h, C.byref(t)))
+
+    def get_tile_ex(self) -> dict:
+        t = L.Tile()
+        L.check(L.lib.oe_scene_get_tile_ex(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, force: bool = False) -> dict:
+        """Tune (or take the cached) colour-pass layout; returns it with the
+        time per sweep and the number of layouts timed (0 = from the cache)."""
+        t, ms, n = L.Tile(), C.c_double(), C.c_int32()
+        L.check(L.lib.oe_scene_autotune(self._h, int(force), C.byref(t), C.byref(ms), C.byref(n)))
+        out = t.as_dict()
+        out.update(sweep_ms=ms.value, candidates=n.value)
+        return out
 
     def get_tile(self):
         """(lanes, quadrature points per lane, mode)."""
@@ -492,6 +514,29 @@ class Batch:
         L.check(L.lib.oe_batch_info(self._h, C.byref(nl), C.byref(vu), C.byref(gb)))
         return {"gather_launches": nl.value, "vertex_updates": vu.value, "gather_bytes": gb.value}
 
+    def set_tile_ex(self, element: int, lanes=64, qpl=1, mode=PASS_GATHER, inc_per_block=0, latency_blocks=512):
+        t = L.Tile(lanes, qpl, int(mode), inc_per_block, latency_blocks)
+        L.check(L.lib.oe_batch_set_tile_ex(self._h, element, C.byref(t)))
+
+    def get_tile_ex(self, element: int) -> dict:
+        t = L.Tile()
+        L.check(L.lib.oe_batch_get_tile_ex(self._h, element, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, force: bool = False):
+        L.check(L.lib.oe_batch_autotune(self._h, int(force)))
+
+    def get_timing_element(self, element: int):
+        """Event-timed colour-pass milliseconds and launches of one element type since its last read."""
+        ms, n = C.c_double(), C.c_int64()
+        L.check(L.lib.oe_batch_get_timing_element(self._h, element, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def info_element(self, element: int):
+        nl, vu, gb = C.c_int32(), C.c_int64(), C.c_double()
+        L.check(L.lib.oe_batch_info_element(self._h, element, C.byref(nl), C.byref(vu), C.byref(gb)))
+        return {"gather_launches": nl.value, "vertex_updates": vu.value, "gather_bytes": gb.value}
+
     def close(self):
         if getattr(self, "_h", None):
             L.lib.oe_batch_destroy(self._h)
@@ -608,3 +653,27 @@ def explicit_scene(engine, element, xyz, topo, sub_it, density, distrib, young, 
               distribution=distrib, solver=SOLVER_EXPLICIT, damping=damping, sub_iterations=sub_it, **kw)
     s._solver = SOLVER_EXPLICIT
     return s
+
+
+def autotune_cache():
+    """The autotuner's cache: one dict per (device, element, size bucket, form)."""
+    n = C.c_int32(0)
+    L.check(L.lib.oe_autotune_cache(C.byref(n), None, None, None, None, None, None, None))
+    cap = n.value
+    dev, el, bk, fm, cand = (np.zeros(cap, np.int32) for _ in range(5))
+    ms = np.zeros(cap, np.float64)
+    tiles = (L.Tile * max(cap, 1))()
+    n = C.c_int32(cap)
+    L.check(L.lib.oe_autotune_cache(C.byref(n), L.iptr(dev), L.iptr(el), L.iptr(bk), L.iptr(fm), tiles,
+                                    ms.ctypes.data_as(C.POINTER(C.c_double)), L.iptr(cand)))
+    out = []
+    for i in range(min(cap, n.value)):
+        d = tiles[i].as_dict()
+        d.update(device=int(dev[i]), element=int(el[i]), bucket=int(bk[i]), form="batch" if fm[i] else "scene",
+                 ms=float(ms[i]), candidates=int(cand[i]))
+        out.append(d)
+    return out
+
+
+def autotune_clear():
+    L.check(L.lib.oe_autotune_clear())

@@ -62,6 +62,20 @@ class SceneDesc(C.Structure):
     ]
 
 
+class Tile(C.Structure):
+    """oe_tile: the full colour-pass layout."""
+    _fields_ = [
+        ("lanes", C.c_int32),
+        ("qpl", C.c_int32),
+        ("mode", C.c_int32),
+        ("inc_per_block", C.c_int32),
+        ("latency_blocks", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 P = C.c_void_p
 f32p = C.POINTER(C.c_float)
 i32p = C.POINTER(C.c_int32)
@@ -107,6 +121,17 @@ _SIGS = {
     "oe_batch_set_timing": (C.c_int32, [P, C.c_int32]),
     "oe_batch_get_timing": (C.c_int32, [P, f64p, i64p]),
     "oe_batch_info": (C.c_int32, [P, i32p, i64p, f64p]),
+    "oe_batch_get_timing_element": (C.c_int32, [P, C.c_int32, f64p, i64p]),
+    "oe_batch_set_tile_ex": (C.c_int32, [P, C.c_int32, C.POINTER(Tile)]),
+    "oe_batch_get_tile_ex": (C.c_int32, [P, C.c_int32, C.POINTER(Tile)]),
+    "oe_batch_autotune": (C.c_int32, [P, C.c_int32]),
+    "oe_scene_set_tile_ex": (C.c_int32, [P, C.POINTER(Tile)]),
+    "oe_scene_get_tile_ex": (C.c_int32, [P, C.POINTER(Tile)]),
+    "oe_scene_autotune": (C.c_int32, [P, C.c_int32, C.POINTER(Tile), f64p, i32p]),
+    "oe_engine_set_autotune": (C.c_int32, [P, C.c_int32]),
+    "oe_autotune_cache": (C.c_int32, [i32p, i32p, i32p, i32p, i32p, C.POINTER(Tile), f64p, i32p]),
+    "oe_autotune_clear": (C.c_int32, []),
+    "oe_batch_info_element": (C.c_int32, [P, C.c_int32, i32p, i64p, f64p]),
     "oe_server_create": (C.c_int32, [P, C.c_int32, C.c_int32, C.POINTER(P)]),
     "oe_server_destroy": (C.c_int32, [P]),
     "oe_server_add_scene": (C.c_int32, [P, P, i32p]),

@@ -64,9 +64,9 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
     d.ju = t.mode == PASS_CACHE ? s.vbd_fem->ensure_ju() : nullptr;
     d.fblocks = d.fblocks_lat = nullptr;
     if (t.mode == PASS_TASKS) {
-        s.vbd_fem->prepare_tasks();
-        d.fblocks = s.vbd_fem->d_fblk.get();
-        d.fblocks_lat = s.vbd_fem->d_fblk_lat.get();
+        d.fblocks = s.vbd_fem->throughput_tables(t.inc).blocks.get();
+        const TaskTables* lt = s.vbd_fem->latency_tables();
+        d.fblocks_lat = lt ? lt->blocks.get() : nullptr;
     }
     d.ps = s.ps->get_parameters();
     d.fem = s.fem->fem_view();
@@ -104,6 +104,8 @@ void Schedule::clear() {
     max_iterations = 0;
     vertex_updates_per_substep = 0;
     algorithmic_bytes_per_substep = 0.0;
+    algorithmic_bytes_by_element.clear();
+    vertex_updates_by_element.clear();
 }
 
 void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& table_index,
@@ -115,28 +117,32 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
         max_iterations = std::max(max_iterations, s->cfg.iterations);
         vertex_updates_per_substep += static_cast<int64_t>(s->ps->n) * s->cfg.iterations;
         const int S = s->fem->S, Q = s->fem->Q;
-        algorithmic_bytes_per_substep +=
+        const double bytes =
             (static_cast<double>(s->fem->E) * S * (40.0 * Q + 16.0 * S + 8.0) + 56.0 * s->ps->n) * s->cfg.iterations;
+        algorithmic_bytes_per_substep += bytes;
+        algorithmic_bytes_by_element[s->mesh.element] += bytes;
+        vertex_updates_by_element[s->mesh.element] += static_cast<int64_t>(s->ps->n) * s->cfg.iterations;
     }
     static const Element order[] = {Tetra, Tetra10, Tetra20, Hexa, Hexa27};
     for (int c = 0; c < max_colors; ++c) {
         for (Element el : order) {
             GLaunch gl{};
             gl.color = c;
+            gl.element = el;
             gl.seg_first = static_cast<int>(segs.size());
             // Task passes take the latency form when it was asked for, every
             // scene of the launch has its vertex blocks, and the throughput
             // form would give fewer than 2 blocks per CU.
-            if (want_latency && tile_of(tiles, el).mode == PASS_TASKS) {
+            const GatherTile et = tile_of(tiles, el);
+            if (want_latency && et.mode == PASS_TASKS) {
                 bool all = true;
                 int64_t through_blocks = 0;
                 for (Scene* s : scenes) {
                     if (s->mesh.element != el || c >= s->vbd_fem->nb_colors()) continue;
-                    if (!s->vbd_fem->d_fblk_lat.get()) all = false;
-                    const std::vector<int32_t>& fb = s->vbd_fem->fblk_color_begin;
-                    if (!fb.empty()) through_blocks += fb[c + 1] - fb[c];
+                    if (!s->vbd_fem->latency_tables()) all = false;
+                    through_blocks += s->vbd_fem->throughput_tables(et.inc).color_blocks(c);
                 }
-                gl.latency = all && through_blocks < kLatencyBlocks;
+                gl.latency = all && through_blocks < et.lat_blocks;
             }
             bool any = false;
             for (size_t i = 0; i < scenes.size(); ++i) {
@@ -157,11 +163,11 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                 }
                 if (gl.mode == PASS_TASKS) {
                     // fused tasks: one block per vertex block of the colour
-                    const std::vector<int32_t>& fb =
-                        gl.latency ? s->vbd_fem->fblk_lat_color_begin : s->vbd_fem->fblk_color_begin;
-                    if (fb.empty()) throw std::logic_error("vertex blocks not prepared");
-                    const int nfb = fb[c + 1] - fb[c];
-                    segs.push_back(BatchSeg{table_index[i], fb[c], nfb, gl.nblocks});
+                    const TaskTables* tt =
+                        gl.latency ? s->vbd_fem->latency_tables() : &s->vbd_fem->throughput_tables(t.inc);
+                    if (!tt) throw std::logic_error("vertex blocks not prepared");
+                    const int nfb = tt->color_blocks(c);
+                    segs.push_back(BatchSeg{table_index[i], tt->color_begin[c], nfb, gl.nblocks});
                     gl.nblocks += nfb;
                 } else {
                     // gather: L lanes per slot
@@ -221,13 +227,15 @@ void DeviceSchedule::upload(Engine* eng, const Schedule& s, hipStream_t stream, 
     if (sync) OE_HIP(hipStreamSynchronize(stream));
 }
 
-void GatherTimer::before(hipStream_t s) {
+void GatherTimer::before(hipStream_t s, int tag) {
     if (!enabled) return;
     while (pool.size() < used + 2) {
         hipEvent_t ev;
         OE_HIP(hipEventCreate(&ev));
         pool.push_back(ev);
     }
+    tags.resize(used / 2 + 1);
+    tags[used / 2] = tag;
     OE_HIP(hipEventRecord(pool[used], s));
 }
 
@@ -245,6 +253,8 @@ void GatherTimer::collect(hipStream_t s) {
         OE_HIP(hipEventElapsedTime(&t, pool[i], pool[i + 1]));
         ms += t;
         launches++;
+        tag_ms[tags[i / 2]] += t;
+        tag_launches[tags[i / 2]]++;
     }
     used = 0;
 }
@@ -275,7 +285,7 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
             for (const Schedule::GLaunch& gl : sch.glaunches) {
                 g.seg_first = gl.seg_first;
                 g.seg_count = gl.seg_count;
-                if (timer) timer->before(stream);
+                if (timer) timer->before(stream, gl.element);
                 if (gl.mode == PASS_TASKS && gl.S == 27 && gl.Q == 27) {
                     tk.seg_first = gl.seg_first;
                     tk.seg_count = gl.seg_count;

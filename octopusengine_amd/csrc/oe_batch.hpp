@@ -33,7 +33,7 @@ namespace oe {
 // is O(segments), not O(blocks), to build and upload.
 struct Schedule {
     struct GLaunch {
-        int color, S, Q, L, QL;
+        int color, element, S, Q, L, QL;
         int mode;                         // PassMode
         bool latency;                     // task pass in the kernels' latency form
         int seg_first, seg_count, nblocks;   // gather: slot ranges; tasks: vertex-block ranges
@@ -55,6 +55,8 @@ struct Schedule {
     int max_iterations = 0;
     int64_t vertex_updates_per_substep = 0;
     double algorithmic_bytes_per_substep = 0.0;   // gather kernels, all iterations
+    std::map<int, double> algorithmic_bytes_by_element;    // the same per element type
+    std::map<int, int64_t> vertex_updates_by_element;
 
     void clear();
     // scenes[i] is placed at SceneDev table index table_index[i]; want_latency
@@ -79,13 +81,18 @@ SceneDev make_scene_dev(Scene& s, const unsigned char* fixed, const std::map<int
 std::vector<unsigned char> fixed_flags(const Scene& s);
 void check_batchable(const Scene& s);
 
+// HIP events around colour-pass launches; durations are summed in total and
+// per tag (the launch's element type).
 struct GatherTimer {
     bool enabled = false;
     std::vector<hipEvent_t> pool;
+    std::vector<int> tags;   // one per event pair
     size_t used = 0;
     double ms = 0.0;
     int64_t launches = 0;
-    void before(hipStream_t s);
+    std::map<int, double> tag_ms;
+    std::map<int, int64_t> tag_launches;
+    void before(hipStream_t s, int tag = 0);
     void after(hipStream_t s);
     void collect(hipStream_t s);
     ~GatherTimer();

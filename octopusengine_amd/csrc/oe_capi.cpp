@@ -12,6 +12,7 @@
 #include "oe_batch.hpp"
 #include "oe_server.hpp"
 #include "oe_scene.hpp"
+#include "oe_tune.hpp"
 
 struct oe_engine {
     std::unique_ptr<oe::Engine> eng;
@@ -205,6 +206,14 @@ oe_status oe_scene_create(oe_engine* e, const oe_scene_desc* d, oe_scene** out) 
             throw;
         }
         s->owner = e;
+        if (e->eng->autotune && (s->scene->vbd_fem || s->scene->explicit_fem)) {
+            try {
+                oe::autotune_scene(*s->scene, false);
+            } catch (...) {
+                delete s;
+                throw;
+            }
+        }
         *out = s;
     });
 }
@@ -300,6 +309,93 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
         }
         if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, oe::PASS_GATHER};
     });
+}
+
+namespace {
+oe::GatherTile tile_from(const oe_tile* t) {
+    oe::GatherTile g{t->lanes, t->qpl, t->mode};
+    g.inc = t->inc_per_block;
+    g.lat_blocks = t->latency_blocks;
+    return g;
+}
+void tile_to(const oe::GatherTile& g, oe_tile* t) {
+    t->lanes = g.L;
+    t->qpl = g.QL;
+    t->mode = g.mode;
+    t->inc_per_block = g.inc;
+    t->latency_blocks = g.lat_blocks;
+}
+void check_tile_ex(int element, const oe_tile* t) {
+    check_tile(element, t->lanes, t->qpl, t->mode);
+    OE_REQUIRE(t->inc_per_block >= 0 && t->inc_per_block <= oe::max_task_inc(static_cast<oe::Element>(element)),
+               "incidences per block must be 0 (the kernel's maximum) or at most the kernel's maximum");
+    OE_REQUIRE(t->latency_blocks >= 0, "latency_blocks must be >= 0");
+}
+}  // namespace
+
+oe_status oe_scene_set_tile_ex(oe_scene* s, const oe_tile* t) {
+    return guard([&] {
+        OE_REQUIRE(s && t, "bad argument");
+        oe::Scene& sc = *s->scene;
+        require_unshared(sc, "set_tile");
+        check_tile_ex(sc.mesh.element, t);
+        OE_REQUIRE(t->mode == oe::PASS_GATHER || sc.vbd_fem, "cached gradients and element tasks are VBD options");
+        OE_REQUIRE(t->mode != oe::PASS_TASKS || sc.mesh.element != oe::Hexa27 || sc.vbd_fem->has_q2_basis,
+                   "no tensor factors for this 27-node hexahedron");
+        oe::apply_scene_tile(sc, tile_from(t));
+    });
+}
+
+oe_status oe_scene_get_tile_ex(oe_scene* s, oe_tile* t) {
+    return guard([&] {
+        OE_REQUIRE(s && t, "bad argument");
+        oe::GatherTile g{0, 0, oe::PASS_GATHER};
+        if (s->scene->vbd_fem) g = s->scene->vbd_fem->tile;
+        if (s->scene->explicit_fem) g = s->scene->explicit_fem->tile;
+        tile_to(g, t);
+    });
+}
+
+oe_status oe_scene_autotune(oe_scene* s, int32_t force, oe_tile* chosen, double* sweep_ms, int32_t* candidates) {
+    return guard([&] {
+        OE_REQUIRE(s, "null scene");
+        require_unshared(*s->scene, "autotune");
+        const oe::TuneRecord r = oe::autotune_scene(*s->scene, force != 0);
+        if (chosen) tile_to(r.tile, chosen);
+        if (sweep_ms) *sweep_ms = r.ms;
+        if (candidates) *candidates = r.candidates;
+    });
+}
+
+oe_status oe_engine_set_autotune(oe_engine* e, int32_t enable) {
+    return guard([&] {
+        OE_REQUIRE(e, "null engine");
+        e->eng->autotune = enable != 0;
+    });
+}
+
+oe_status oe_autotune_cache(int32_t* n, int32_t* device, int32_t* element, int32_t* bucket, int32_t* form,
+                            oe_tile* tiles, double* ms, int32_t* candidates) {
+    return guard([&] {
+        OE_REQUIRE(n && *n >= 0, "bad capacity");
+        const std::vector<oe::TuneRecord> recs = oe::tune_cache_records();
+        const int32_t cap = *n;
+        *n = static_cast<int32_t>(recs.size());
+        for (int32_t i = 0; i < cap && i < static_cast<int32_t>(recs.size()); ++i) {
+            const oe::TuneRecord& r = recs[i];
+            if (device) device[i] = r.device;
+            if (element) element[i] = r.element;
+            if (bucket) bucket[i] = r.bucket;
+            if (form) form[i] = r.form;
+            if (tiles) tile_to(r.tile, &tiles[i]);
+            if (ms) ms[i] = r.ms;
+            if (candidates) candidates[i] = r.candidates;
+        }
+    });
+}
+
+oe_status oe_autotune_clear(void) {
+    return guard([&] { oe::tune_cache_clear(); });
 }
 
 oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* mode) {
@@ -467,6 +563,29 @@ oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t
     });
 }
 
+oe_status oe_batch_set_tile_ex(oe_batch* b, int32_t element, const oe_tile* t) {
+    return guard([&] {
+        OE_REQUIRE(b && t && oe::element_supported(element), "bad argument");
+        check_tile_ex(element, t);
+        b->batch->set_tile(static_cast<oe::Element>(element), tile_from(t));
+    });
+}
+
+oe_status oe_batch_get_tile_ex(oe_batch* b, int32_t element, oe_tile* t) {
+    return guard([&] {
+        OE_REQUIRE(b && t && oe::element_supported(element), "bad argument");
+        const auto it = b->batch->tiles.find(element);
+        tile_to(it != b->batch->tiles.end() ? it->second : oe::default_tile(static_cast<oe::Element>(element)), t);
+    });
+}
+
+oe_status oe_batch_autotune(oe_batch* b, int32_t force) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        (void)oe::autotune_batch(*b->batch, force != 0);
+    });
+}
+
 oe_status oe_batch_set_timing(oe_batch* b, int32_t enable) {
     return guard([&] {
         OE_REQUIRE(b, "null batch");
@@ -483,6 +602,33 @@ oe_status oe_batch_get_timing(oe_batch* b, double* gather_ms, int64_t* launches)
         if (launches) *launches = t.launches;
         t.ms = 0.0;
         t.launches = 0;
+    });
+}
+
+oe_status oe_batch_get_timing_element(oe_batch* b, int32_t element, double* gather_ms, int64_t* launches) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        oe::GatherTimer& t = b->batch->timer;
+        t.collect(b->batch->eng->stream);
+        if (gather_ms) *gather_ms = t.tag_ms[element];
+        if (launches) *launches = t.tag_launches[element];
+        t.tag_ms.erase(element);
+        t.tag_launches.erase(element);
+    });
+}
+
+oe_status oe_batch_info_element(oe_batch* b, int32_t element, int32_t* gather_launches, int64_t* vertex_updates,
+                                double* gather_bytes) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        const oe::Schedule& s = b->batch->sched;
+        int32_t n = 0;
+        for (const auto& gl : s.glaunches) n += gl.element == element ? 1 : 0;
+        const auto vu = s.vertex_updates_by_element.find(element);
+        const auto by = s.algorithmic_bytes_by_element.find(element);
+        if (gather_launches) *gather_launches = n * s.max_iterations;
+        if (vertex_updates) *vertex_updates = vu == s.vertex_updates_by_element.end() ? 0 : vu->second;
+        if (gather_bytes) *gather_bytes = by == s.algorithmic_bytes_by_element.end() ? 0.0 : by->second;
     });
 }
 

@@ -302,19 +302,42 @@ void upload_vertex_blocks(Engine* eng, const OwnersCSR& o, int max_inc, int max_
 }
 }  // namespace
 
-void GPU_VBD_FEM::prepare_tasks() {
-    if (d_fblk.get()) return;
-    const bool q2 = element == Hexa27;
-    upload_vertex_blocks(eng, owners, q2 ? kQ2FusedIncPerBlock : kFusedIncPerBlock,
-                         q2 ? kQ2FusedMaxSlots : kFusedMaxSlots, d_fblk, fblk_color_begin);
-    try {
-        upload_vertex_blocks(eng, owners, q2 ? kQ2FusedIncPerBlockLat : kFusedIncPerBlockLat,
-                             q2 ? kQ2FusedMaxSlots : kFusedMaxSlots, d_fblk_lat, fblk_lat_color_begin);
-    } catch (const std::invalid_argument&) {
-        // a vertex with too many incidences for a latency block: throughput form only
-        d_fblk_lat = DBuf<int4>();
-        fblk_lat_color_begin.clear();
+const TaskTables& GPU_VBD_FEM::throughput_tables(int inc) {
+    const int maxi = max_task_inc(element);
+    const int n = inc > 0 ? inc : maxi;
+    if (n > maxi) throw std::invalid_argument("more incidences per block than the task kernel holds");
+    auto it = thr_tables.find(n);
+    if (it != thr_tables.end()) return it->second;
+    TaskTables t;
+    upload_vertex_blocks(eng, owners, n, element == Hexa27 ? kQ2FusedMaxSlots : kFusedMaxSlots, t.blocks,
+                         t.color_begin);
+    return thr_tables.emplace(n, std::move(t)).first->second;
+}
+
+const TaskTables& GPU_VBD_FEM::built_tables(int inc) const {
+    const auto it = thr_tables.find(inc > 0 ? inc : max_task_inc(element));
+    if (it == thr_tables.end()) throw std::logic_error("element-task vertex blocks not prepared");
+    return it->second;
+}
+
+const TaskTables* GPU_VBD_FEM::latency_tables() {
+    if (!lat_built) {
+        lat_built = true;
+        const bool q2 = element == Hexa27;
+        try {
+            upload_vertex_blocks(eng, owners, q2 ? kQ2FusedIncPerBlockLat : kFusedIncPerBlockLat,
+                                 q2 ? kQ2FusedMaxSlots : kFusedMaxSlots, lat_tables.blocks, lat_tables.color_begin);
+        } catch (const std::invalid_argument&) {
+            // a vertex with too many incidences for a latency block: throughput form only
+            lat_tables = TaskTables();
+        }
     }
+    return lat_tables.blocks.get() ? &lat_tables : nullptr;
+}
+
+void GPU_VBD_FEM::prepare_tasks() {
+    throughput_tables(tile.inc);
+    latency_tables();
 }
 
 float* GPU_VBD_FEM::ensure_ju() {
@@ -357,15 +380,34 @@ void GPU_VBD_FEM::residual(GPU_ParticleSystem* ps, float dt, std::vector<float>&
 
 void GPU_VBD_FEM::set_shuffle_seed(uint64_t seed) {
     shuffle = seed != 0;
-    rng.seed(seed);
+    rng_state = seed;
+}
+
+namespace {
+// SplitMix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014): add the golden-ratio increment, then mix.
+uint64_t splitmix64(uint64_t& state) {
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+void GPU_VBD_FEM::next_color_order(std::vector<int>& order) {
+    order.resize(colors.nb_color);
+    std::iota(order.begin(), order.end(), 0);
+    if (!shuffle) return;
+    for (int i = colors.nb_color - 1; i > 0; --i) {
+        const int j = static_cast<int>(splitmix64(rng_state) % static_cast<uint64_t>(i + 1));
+        std::swap(order[i], order[j]);
+    }
 }
 
 void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
     if (!y) throw std::logic_error("GPU_VBD_FEM used outside a GPU_VBD integrator");
-    const int nc = colors.nb_color;
-    std::vector<int> order(nc);
-    std::iota(order.begin(), order.end(), 0);
-    if (shuffle) std::shuffle(order.begin(), order.end(), rng);
+    std::vector<int> order;
+    next_color_order(order);
     GatherArgs a{};
     a.ps = ps->get_parameters();
     a.fem = fem_view();
@@ -386,7 +428,8 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
         if (!launch_ju_refresh(S, Q, a.ps, a.fem, a.ju, s)) throw std::invalid_argument("unsupported element");
     }
     const bool by_tasks = tasks();
-    if (by_tasks && !d_fblk.get()) throw std::logic_error("element-task vertex blocks not prepared");
+    const TaskTables* thr = by_tasks ? &built_tables(tile.inc) : nullptr;
+    const TaskTables* lt = by_tasks && lat_tables.blocks.get() ? &lat_tables : nullptr;
     for (int c : order) {
         a.first_slot = owners.color_begin[c];
         a.count = owners.color_begin[c + 1] - a.first_slot;
@@ -399,11 +442,12 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
             OE_HIP(hipEventRecord(ev_pool[ev_used], s));
         }
         if (by_tasks) {
-            // the latency form for colours under 2 throughput-form blocks per CU
-            const bool lat = d_fblk_lat.get() && fblk_color_begin[c + 1] - fblk_color_begin[c] < kLatencyBlocks;
-            const std::vector<int32_t>& cb = lat ? fblk_lat_color_begin : fblk_color_begin;
-            const int4* blocks = lat ? d_fblk_lat.get() : d_fblk.get();
-            const int b0 = cb[c], nb = cb[c + 1] - cb[c];
+            // the latency form for colours with few throughput-form blocks
+            // (by default under 2 per CU)
+            const bool lat = lt && thr->color_blocks(c) < tile.lat_blocks;
+            const TaskTables& tt = lat ? *lt : *thr;
+            const int4* blocks = tt.blocks.get();
+            const int b0 = tt.color_begin[c], nb = tt.color_blocks(c);
             if (element == Hexa27)
                 launch_q2_fused(Q2FusedArgs{a.ps, a.fem, a.own, a.mt, q2_basis, y, blocks, damping, dt, b0, nb,
                                             lat ? 1 : 0},

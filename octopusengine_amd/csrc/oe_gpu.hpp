@@ -20,12 +20,15 @@
 
 #include <cstdint>
 #include <memory>
-#include <random>
 #include <vector>
+
+#include <map>
 
 #include "oe_device.hpp"
 #include "oe_elementwise.hpp"
+#include "oe_gather.hpp"
 #include "oe_host.hpp"
+#include "oe_q2.hpp"
 
 namespace oe {
 
@@ -33,6 +36,7 @@ struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     MemoryPool pool;
+    bool autotune = false;   // tune the layout of every VBD / explicit scene created
     explicit Engine(int dev, size_t budget_bytes = SIZE_MAX);
     ~Engine();
     Engine(const Engine&) = delete;
@@ -138,15 +142,36 @@ struct GatherTile {
     int L = 16;
     int QL = 1;
     int mode = PASS_GATHER;
+    // element tasks: incidences per throughput-form block (the elements one
+    // workgroup stages in LDS; 0 = the kernel's maximum), and the number of
+    // throughput-form blocks below which a colour takes the latency form
+    int inc = 0;
+    int64_t lat_blocks = kLatencyBlocks;
 };
 GatherTile default_tile(Element e);
+// Largest incidences per block the element-task kernels hold.
+inline int max_task_inc(Element e) { return e == Hexa27 ? kQ2FusedIncPerBlock : kFusedIncPerBlock; }
+
+// Vertex blocks of the fused element-task kernels: int4 {first slot, slot
+// count, first owners index, incidence count} per block, colour-major;
+// color_begin[c] = first block of colour c.
+struct TaskTables {
+    DBuf<int4> blocks;
+    std::vector<int32_t> color_begin;
+    int32_t color_blocks(int c) const { return color_begin[c + 1] - color_begin[c]; }
+};
 
 struct GPU_VBD_FEM final : GPU_FEM {
     GPU_VBD_FEM(Engine* eng, Element element, const HostMesh& mesh, Material material, float young, float poisson,
                 float damping, VBD_Version v = Reduction_Symmetry, int target_color = -1);
     void step(GPU_ParticleSystem* ps, float dt) override;
-    // colour order of the next sweep: ascending, or shuffled from a seed
+    // Colour order of every sweep: ascending (seed 0), or a Fisher-Yates
+    // shuffle of 0..nc-1 per sweep (i = nc-1 down to 1 swaps i with
+    // SplitMix64() % (i + 1)) driven by one SplitMix64 stream seeded once.
+    // The reference shuffles with a time-seeded generator every iteration
+    // (src/GPU/VBD/GPU_VBD_FEM.cu:713-716); a fixed seed makes it repeatable.
     void set_shuffle_seed(uint64_t seed);
+    void next_color_order(std::vector<int>& order);
     int nb_colors() const { return colors.nb_color; }
 
     Coloration colors;
@@ -156,7 +181,7 @@ struct GPU_VBD_FEM final : GPU_FEM {
     float damping;
     GatherTile tile;
     bool shuffle = false;
-    std::mt19937_64 rng;
+    uint64_t rng_state = 0;   // SplitMix64 state
     const float4* y = nullptr;   // bound by GPU_VBD::add_dynamics
     DBuf<int> d_slot_vid;
     DBuf<int2> d_slot_nbo;
@@ -169,16 +194,23 @@ struct GPU_VBD_FEM final : GPU_FEM {
     void residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r);
     DBuf<float4> d_residual;
     bool cached() const { return tile.mode == PASS_CACHE && version != Better_Coloration; }
-    // vertex blocks of the fused element-task kernels (4 ints per block,
-    // colour-major; fblk_color_begin[c] = first block of colour c), made by
-    // prepare_tasks()
-    DBuf<int4> d_fblk;
-    std::vector<int32_t> fblk_color_begin;
-    // vertex blocks of the kernels' latency form (one task per thread or
-    // half-wave); absent when a vertex has more incidences than one holds
-    DBuf<int4> d_fblk_lat;
-    std::vector<int32_t> fblk_lat_color_begin;
-    void prepare_tasks();
+    // Vertex blocks of the throughput form, one table per block size
+    // (incidences per block), built on first use and kept while the scene
+    // lives: batches and servers hold pointers into them.
+    std::map<int, TaskTables> thr_tables;
+    // Vertex blocks of the kernels' latency form (one task per thread or
+    // half-wave); empty when a vertex has more incidences than one holds.
+    TaskTables lat_tables;
+    bool lat_built = false;
+    // Build (if needed) and return the throughput tables for `inc`
+    // incidences per block (0 = the kernel's maximum); throws
+    // std::invalid_argument when a vertex has more incidences than a block.
+    const TaskTables& throughput_tables(int inc);
+    // The tables already built for `inc` (std::logic_error if not): for
+    // step(), which may run under HIP graph capture and must not allocate.
+    const TaskTables& built_tables(int inc) const;
+    const TaskTables* latency_tables();   // null when absent
+    void prepare_tasks();                 // the tables of the current tile
     bool tasks() const {
         return tile.mode == PASS_TASKS && version != Better_Coloration && (element != Hexa27 || has_q2_basis);
     }

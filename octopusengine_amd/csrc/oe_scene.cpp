@@ -131,9 +131,11 @@ void Scene::step(float dt, int substeps) {
 
 size_t Scene::device_bytes() const {
     size_t b = ps->device_bytes() + fem->topo.bytes() + fem->jxv.bytes() + fem->dN.bytes() + fem->wq.bytes();
-    if (vbd_fem)
+    if (vbd_fem) {
         b += vbd_fem->d_slot_vid.bytes() + vbd_fem->d_slot_nbo.bytes() + vbd_fem->d_own.bytes() +
-             vbd_fem->p_new.bytes() + vbd_fem->ju.bytes() + vbd_fem->d_fblk.bytes() + vbd_fem->d_fblk_lat.bytes();
+             vbd_fem->p_new.bytes() + vbd_fem->ju.bytes() + vbd_fem->lat_tables.blocks.bytes();
+        for (const auto& kv : vbd_fem->thr_tables) b += kv.second.blocks.bytes();
+    }
     return b;
 }
 

@@ -49,6 +49,9 @@ struct Scene {
     void box_limit(const float pmin[3], const float pmax[3]);
     void step(float dt, int substeps);
     void set_graph(bool on);
+    // Drop the captured substep graph (kept enabled): the next step captures
+    // the launch sequence again, e.g. after a tile change.
+    void invalidate_graph() { drop_graph(); }
     size_t device_bytes() const;
 
     Engine* eng;

@@ -115,3 +115,50 @@ def test_writer_data_blocks_and_errors(tmp_path):
     assert lines[i + 5] == "0.5 0.5 0.5  " * xyz.shape[0]
     j = lines.index(f"CELL_DATA {topo.shape[0]}")
     assert lines[j + 3].split() == ["2.5e-07"] * topo.shape[0]
+
+
+# The reference's larger result files (3_*, 4_* and the ref_* solutions):
+# SHA-256 of the mesh sections, the beam's z offset in its scene, and the
+# weights (float32), written by tests/golden/make_vtk_fixtures.py.  These pin
+# the converters' numbering and the masses at the 32 x 8 x 8 Q2 / P2 and
+# 64 x 16 x 16 Q1 sizes of the workloads.
+LARGE = np.load(Path(__file__).resolve().parent / "golden" / "vtk_vbd_scaling_large.npz")
+LARGE_NAMES = [str(n) for n in LARGE["names"]]
+
+
+def test_large_fixture_covers_the_workload_meshes():
+    assert {"4_Hexa27_32_8_8", "4_Tetra10_32_8_8", "4_Hexa_64_16_16"} <= set(LARGE_NAMES)
+    assert len(LARGE_NAMES) == 10
+
+
+@pytest.mark.parametrize("name", LARGE_NAMES)
+def test_engine_reproduces_large_reference_mesh_digest(name, tmp_path):
+    import hashlib
+
+    el = int(LARGE[f"{name}__element"])
+    cells = tuple(int(c) for c in LARGE[f"{name}__cells"])
+    xyz, topo = oe.beam_mesh(el, cells, SIZE)
+    xyz = (xyz + np.float32(float(LARGE[f"{name}__zoff"]) - xyz[0, 2]) * np.array([0, 0, 1], np.float32))
+    path = tmp_path / "beam.vtk"
+    oe.VtkWriter(path).mesh(el, xyz.astype(np.float32), topo).close()
+    digest = hashlib.sha256(_sections(path.read_text()).encode()).hexdigest()
+    assert digest == str(LARGE[f"{name}__mesh_sha256"])
+
+
+@pytest.mark.parametrize("name", LARGE_NAMES)
+def test_masses_match_large_reference_weights(name):
+    el = int(LARGE[f"{name}__element"])
+    cells = tuple(int(c) for c in LARGE[f"{name}__cells"])
+    weights = LARGE[f"{name}__weights"].astype(np.float64)
+    xyz, topo = oe.beam_mesh(el, cells, SIZE)
+    m = oe.host_masses(el, xyz, topo, 1000.0, oe.MASS_FEMSHAPE)
+    assert m.shape == weights.shape
+    # The reference builds J = sum_k X_k (x) dN_k in float from coordinates of
+    # magnitude up to 4 + |z offset| at element size h, so det J (and the
+    # weights) carry a relative rounding error up to about
+    # 3 (4 + |z|) 2^-24 / h; the files also keep 6 significant digits.
+    h = SIZE[0] / cells[0]
+    rel = max(5e-6, 3.0 * (4.0 + abs(float(LARGE[f"{name}__zoff"]))) * 2.0 ** -24 / h)
+    assert np.abs(m - weights).max() <= rel * weights.max()
+    mo = O.masses(el, xyz, topo, 1000.0, True)
+    assert np.abs(mo - weights).max() <= rel * weights.max()
