@@ -291,6 +291,15 @@ oe_status oe_scene_element_diagnostics(oe_scene* s, float* stress, float* volume
  * (3 * n_vertices floats; fixed vertices 0).  Replaces GPU_VBD_FEM::get_forces
  * (src/GPU/VBD/GPU_VBD_FEM.cu:482-559). */
 oe_status oe_scene_residual(oe_scene* s, float dt, float* r);
+/* The same residual with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its kernel writes into a static device array of 3 floats per vertex
+ * (scenes of at most 524288 vertices, else OE_ERR_INVALID; 6 MiB held on each
+ * device the library runs on until the process ends) that every call on one
+ * device shares: calls that overlap on one device race.  Results equal
+ * oe_scene_residual's.  Synchronising. */
+oe_status oe_scene_residual_scratch(oe_scene* s, float dt, float* r);
 
 /* ------------------------------------------------------------ mesh files */
 /* Legacy-VTK ASCII unstructured grids (.vtk) and Gmsh 2.2 ASCII (.msh), as the

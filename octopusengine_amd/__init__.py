@@ -405,6 +405,12 @@ class Scene:
         L.check(L.lib.oe_scene_residual(self._h, dt, L.fptr(r)))
         return r
 
+    def residual_scratch(self, dt: float):
+        """The same residual through the allocation-free entry point (static device scratch)."""
+        r = np.empty((self.n_vertices, 3), np.float32)
+        L.check(L.lib.oe_scene_residual_scratch(self._h, dt, L.fptr(r)))
+        return r
+
     def save_vtk(self, path):
         """Rest geometry + point data "weights" (masses) and "u" (displacement)."""
         L.check(L.lib.oe_scene_save_vtk(self._h, str(path).encode()))

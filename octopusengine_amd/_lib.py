@@ -150,6 +150,7 @@ _SIGS = {
     "oe_host_masses": (C.c_int32, [C.c_int32, f32p, C.c_int32, i32p, C.c_int32, C.c_float, C.c_int32, f32p]),
     "oe_scene_element_diagnostics": (C.c_int32, [P, f32p, f32p, f32p, f32p]),
     "oe_scene_residual": (C.c_int32, [P, C.c_float, f32p]),
+    "oe_scene_residual_scratch": (C.c_int32, [P, C.c_float, f32p]),
     "oe_mesh_load": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "oe_mesh_info": (C.c_int32, [P, i32p, i32p]),
     "oe_mesh_vertices": (C.c_int32, [P, f32p]),

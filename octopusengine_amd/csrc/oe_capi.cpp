@@ -12,6 +12,7 @@
 #include "oe_batch.hpp"
 #include "oe_server.hpp"
 #include "oe_scene.hpp"
+#include "oe_scratch.hpp"
 #include "oe_tune.hpp"
 
 struct oe_engine {
@@ -143,6 +144,7 @@ oe_status oe_engine_create(int32_t device, uint64_t budget_bytes, oe_engine** ou
         auto* e = new oe_engine();
         try {
             e->eng = std::make_unique<oe::Engine>(device, budget_bytes ? budget_bytes : SIZE_MAX);
+            oe::scratch_setup(device);
         } catch (...) {
             delete e;
             throw;
@@ -831,6 +833,15 @@ oe_status oe_scene_residual(oe_scene* s, float dt, float* r) {
         std::vector<float> h;
         s->scene->vbd_fem->residual(s->scene->ps.get(), dt, h);
         std::memcpy(r, h.data(), h.size() * sizeof(float));
+    });
+}
+
+oe_status oe_scene_residual_scratch(oe_scene* s, float dt, float* r) {
+    return guard([&] {
+        OE_REQUIRE(s && r, "bad argument");
+        OE_REQUIRE(s->scene->vbd_fem, "the residual is defined for VBD scenes");
+        wait_served(*s->scene);
+        oe::residual_scratch(*s->scene, dt, r);
     });
 }
 

@@ -152,8 +152,16 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
         return;
     }
     if (a.mode == GATHER_VBD_RESIDUAL) {
-        if (lane == 0)
-            a.p_new[vid] = vbd_residual(acc, P[vid], a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt);
+        if (lane == 0) {
+            const float4 rv = vbd_residual(acc, P[vid], a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt);
+            if (a.out3) {
+                a.out3[3 * static_cast<size_t>(vid)] = rv.x;
+                a.out3[3 * static_cast<size_t>(vid) + 1] = rv.y;
+                a.out3[3 * static_cast<size_t>(vid) + 2] = rv.z;
+            } else {
+                a.p_new[vid] = rv;
+            }
+        }
         return;
     }
     float du[3] = {0.f, 0.f, 0.f};
@@ -233,6 +241,7 @@ __global__ __launch_bounds__(256, gather_min_waves<QL>()) void k_batch_gather(Ba
     a.first_slot = seg.first;
     a.count = seg.count;
     a.mode = GATHER_VBD_INPLACE;
+    a.out3 = nullptr;
     const int local = (blk - seg.block_begin) * 256 + threadIdx.x;
     gather_body<S, Q, L, QL, CACHED>(a, local / L, local & (L - 1));
 }

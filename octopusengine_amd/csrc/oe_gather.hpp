@@ -34,6 +34,9 @@ struct GatherArgs {
     int first_slot;
     int count;
     int mode;
+    // Residual mode only: when set, the residual of vertex v goes to
+    // out3[3v .. 3v + 2] (packed xyz) instead of p_new[v].
+    float* out3;
 };
 
 // Everything a batched kernel needs to know about one scene; an array of these

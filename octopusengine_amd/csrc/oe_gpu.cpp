@@ -345,29 +345,35 @@ float* GPU_VBD_FEM::ensure_ju() {
     return ju.get();
 }
 
-void GPU_VBD_FEM::residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r) {
+void GPU_VBD_FEM::launch_residual(GPU_ParticleSystem* ps, float dt, float4* out4, float* out3) {
     if (!y) throw std::logic_error("GPU_VBD_FEM used outside a GPU_VBD integrator");
     if (!(dt > 0.f)) throw std::invalid_argument("residual needs dt > 0");
-    const int n = ps->nb_particles();
-    if (!d_residual.get()) {
-        d_residual.allocate(&eng->pool, n);
-        OE_HIP(hipMemsetAsync(d_residual.get(), 0, sizeof(float4) * n, eng->stream));
-    }
     GatherArgs a{};
     a.ps = ps->get_parameters();
     a.fem = fem_view();
     a.own = OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()};
     a.mt = mt;
     a.y = y;
-    a.p_new = d_residual.get();
+    a.p_new = out4;
+    a.out3 = out3;
     a.ju = nullptr;
     a.damping = damping;
     a.dt = dt;
     a.first_slot = 0;
-    a.count = n;
+    a.count = ps->nb_particles();
     a.mode = GATHER_VBD_RESIDUAL;
     if (!launch_gather(S, Q, tile.L, tile.QL, a, eng->stream)) throw std::invalid_argument("unsupported gather tile");
     OE_LAUNCH_CHECK();
+}
+
+void GPU_VBD_FEM::residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r) {
+    if (!(dt > 0.f)) throw std::invalid_argument("residual needs dt > 0");
+    const int n = ps->nb_particles();
+    if (!d_residual.get()) d_residual.allocate(&eng->pool, n);
+    // fixed vertices are not written: clear every call (the fixed set may
+    // have changed since the last one)
+    OE_HIP(hipMemsetAsync(d_residual.get(), 0, sizeof(float4) * n, eng->stream));
+    launch_residual(ps, dt, d_residual.get(), nullptr);
     std::vector<float4> h;
     d_residual.download(h, eng->stream);
     r.resize(3 * static_cast<size_t>(n));

@@ -192,6 +192,10 @@ struct GPU_VBD_FEM final : GPU_FEM {
     // Residual force of every free vertex's local problem at the current
     // positions (3N floats; fixed vertices keep 0; synchronising).
     void residual(GPU_ParticleSystem* ps, float dt, std::vector<float>& r);
+    // Enqueue the residual kernel only: float4 per vertex into out4, or
+    // packed xyz into out3 when out3 is set; fixed vertices are not written.
+    // Allocates nothing.
+    void launch_residual(GPU_ParticleSystem* ps, float dt, float4* out4, float* out3);
     DBuf<float4> d_residual;
     bool cached() const { return tile.mode == PASS_CACHE && version != Better_Coloration; }
     // Vertex blocks of the throughput form, one table per block size
