@@ -481,7 +481,20 @@ def served_leg(args) -> dict:
     wall = time.perf_counter() - t_start
     st = srv.stats()
     lat = srv.latencies()[n_lat0:]
+    lat_slot = srv.latency_slots()[n_lat0:]
     vu = st["vertex_updates"] - base["vertex_updates"]
+    # the tail by scene: element type and size class of each request's scene
+    els = np.array([sp[0] for sp in specs])[lat_slot]
+    nvs = nv[lat_slot]
+    by_class = []
+    for el, name in ((oe.TETRA10, "P2"), (oe.HEXA27, "Q2")):
+        for lo, hi in ((0, 25_000), (25_000, 100_000), (100_000, 10**9)):
+            sel = (els == el) & (nvs >= lo) & (nvs < hi)
+            if sel.any():
+                by_class.append({"element": name, "vertices": [lo, min(hi, int(nv.max()))], "requests": int(sel.sum()),
+                                 "p50_ms": float(np.percentile(lat[sel], 50)),
+                                 "p99_ms": float(np.percentile(lat[sel], 99))})
+    slowest = lat >= np.percentile(lat, 99)
     out = {
         "pool_scenes": len(scenes), "pool_vertices": int(nv.sum()), "median_scene_vertices": float(np.median(nv)),
         "max_scene_vertices": int(nv.max()), "offered_rate_req_s": args.serve_rate, "requests": n_req,
@@ -490,6 +503,9 @@ def served_leg(args) -> dict:
         "value": vu / wall, "unit": "vertex-updates/s",
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_ms": float(lat.max()), "ticks": st["batches"] - base["batches"], "mean_batch": st["mean_batch"],
+        "by_scene_class": by_class,
+        "slowest_1pct_share_q2": float((els[slowest] == oe.HEXA27).mean()),
+        "slowest_1pct_median_vertices": float(np.median(nvs[slowest])),
         "max_batch": args.serve_max_batch, "streams": args.serve_streams,
         "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "setup_s": setup_s,
         "device_bytes": int(eng.memory()[0]), "budget_bytes": int(eng.memory()[1]),

@@ -253,6 +253,8 @@ oe_status oe_server_stats(oe_server* sv, int64_t* completed, double* p50_ms, dou
                           int64_t* vertex_updates, int64_t* batches, double* mean_batch);
 /* Per-request latencies in completion order (ms), up to cap entries. */
 oe_status oe_server_latencies(oe_server* sv, double* out, int64_t cap, int64_t* n);
+/* The slot of each request in the same order (for latency by scene). */
+oe_status oe_server_latency_slots(oe_server* sv, int32_t* out, int64_t cap, int64_t* n);
 
 /* ------------------------------------------------- host-side (no GPU needed) */
 /* Beam of nx*ny*nz cells of size sx*sy*sz; pass NULL arrays to query sizes. */

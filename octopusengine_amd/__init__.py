@@ -603,6 +603,14 @@ class Server:
         L.check(L.lib.oe_server_latencies(self._h, L.dptr(out), n.value, C.byref(n)))
         return out
 
+    def latency_slots(self):
+        """Slot of each request, in the order of latencies()."""
+        n = C.c_int64()
+        L.check(L.lib.oe_server_latency_slots(self._h, None, 0, C.byref(n)))
+        out = np.empty(n.value, np.int32)
+        L.check(L.lib.oe_server_latency_slots(self._h, L.iptr(out), n.value, C.byref(n)))
+        return out
+
     def close(self):
         if getattr(self, "_h", None):
             L.lib.oe_server_destroy(self._h)

@@ -141,6 +141,7 @@ _SIGS = {
     "oe_server_poll": (C.c_int32, [P, C.c_uint64, i32p]),
     "oe_server_stats": (C.c_int32, [P, i64p, f64p, f64p, f64p, i64p, i64p, f64p]),
     "oe_server_latencies": (C.c_int32, [P, f64p, C.c_int64, i64p]),
+    "oe_server_latency_slots": (C.c_int32, [P, i32p, C.c_int64, i64p]),
     "oe_host_beam_mesh": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
                                       i32p, i32p, f32p, i32p]),
     "oe_host_shape": (C.c_int32, [C.c_int32, i32p, i32p, f64p, f64p, f64p]),

@@ -727,6 +727,16 @@ oe_status oe_server_latencies(oe_server* sv, double* out, int64_t cap, int64_t* 
     });
 }
 
+oe_status oe_server_latency_slots(oe_server* sv, int32_t* out, int64_t cap, int64_t* n) {
+    return guard([&] {
+        OE_REQUIRE(sv && cap >= 0, "bad argument");
+        const std::vector<int> v = sv->server->latency_slots();
+        const int64_t k = std::min<int64_t>(cap, static_cast<int64_t>(v.size()));
+        if (out && k > 0) std::memcpy(out, v.data(), static_cast<size_t>(k) * sizeof(int32_t));
+        if (n) *n = static_cast<int64_t>(v.size());
+    });
+}
+
 // ------------------------------------------------------------------ host side
 oe_status oe_host_beam_mesh(int32_t element, int32_t nx, int32_t ny, int32_t nz, float sx, float sy, float sz,
                             int32_t* n_vertices, int32_t* n_elements, float* xyz, int32_t* topology) {

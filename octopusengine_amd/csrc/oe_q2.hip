@@ -92,16 +92,25 @@ __device__ __forceinline__ float dpp_add(float x) {
     return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
 }
 
-// Sum over each 32-lane half; the half's total ends in its lanes 16-31.
-// Within each 16-lane row: swap neighbours (quad_perm 1,0,3,2), pairs
-// (quad_perm 2,3,0,1), mirror in 8 lanes, mirror in 16 lanes; then row 1
-// (row 3) adds lane 15 (lane 47) of the row before it (row_bcast:15).
-__device__ __forceinline__ float half_sum_to_upper(float x) {
+// Sum over a 16-lane row (quad_perm 1,0,3,2; quad_perm 2,3,0,1; mirror in 8
+// lanes; mirror in 16 lanes): every lane of the row ends with the row total.
+__device__ __forceinline__ float row_sum(float x) {
     x = dpp_add<0xB1>(x);
     x = dpp_add<0x4E>(x);
     x = dpp_add<0x141>(x);
-    x = dpp_add<0x140>(x);
-    return dpp_add<0x142, 0xA>(x);
+    return dpp_add<0x140>(x);
+}
+
+// Totals of a and b over each 32-lane half (two 16-lane rows).
+// v_permlane16_swap exchanges the second row of its first operand with the
+// first row of its second, so the sum of the two results holds a's partials
+// of both rows (added lane by lane) in the half's first row and b's in its
+// second; a row sum then leaves a's half total in every lane of the first row
+// and b's in every lane of the second.  Half the cross-lane work of two
+// separate half sums.
+__device__ __forceinline__ float half_sum_pair(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return row_sum(__uint_as_float(r[0]) + __uint_as_float(r[1]));
 }
 
 // Lane-constant part of a half's work: its point's tensor indices, the 1D
@@ -191,7 +200,7 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
 // has fewer incidences).  The loads run ahead of the work: task t's
 // displacement, the next task's node index and the one after's owners entry
 // are issued before task t's stages; indices past the block's last incidence
-// are clamped to it.  Partials go to part[task] (lanes 16-24, one float each).
+// are clamped to it.  Partials go to part[task] (nine lanes, one float each).
 __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, const MaterialView& mt,
                                            const LanePlan& P, const int* __restrict__ own, int first, int count,
                                            int task0, TaskLDS& sh, float (*part)[9]) {
@@ -210,13 +219,17 @@ __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, 
         Acc9 acc;
         acc_zero(acc);
         q2_task_body(f, mt, P, eid, r, u, sh, acc);
+        // the half's totals: value 2k in row 0 and value 2k + 1 in row 1 of s[k]
+        float s[5];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc.v[i] = half_sum_to_upper(acc.v[i]);
-        const int i = P.l - 16;
-        if (i >= 0 && i < 9) {
-            float v = acc.v[0];
+        for (int k = 0; k < 4; ++k) s[k] = half_sum_pair(acc.v[2 * k], acc.v[2 * k + 1]);
+        s[4] = half_sum_pair(acc.v[8], 0.f);
+        // lane k of row w writes value 2k + w
+        const int k = P.l & 15, i = 2 * k + (P.l >> 4);
+        if (k < 5 && i < 9) {
+            float v = s[0];
 #pragma unroll
-            for (int k = 1; k < 9; ++k) v = i == k ? acc.v[k] : v;
+            for (int j = 1; j < 5; ++j) v = k == j ? s[j] : v;
             part[task][i] = v;
         }
     }

@@ -148,6 +148,7 @@ void Server::retire(bool) {
         for (Tick& t : done) {
             for (const Request& r : t.reqs) {
                 lat_ms_.push_back(std::chrono::duration<double, std::milli>(now - r.arrival).count());
+                lat_slot_.push_back(r.slot);
                 scene_busy_[r.slot] = 0;
                 slot_open_[r.slot]--;
                 open_tickets_.erase(r.ticket);
@@ -275,6 +276,11 @@ Server::Stats Server::stats() {
 std::vector<double> Server::latencies() {
     std::lock_guard<std::mutex> lk(mu_);
     return lat_ms_;
+}
+
+std::vector<int> Server::latency_slots() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return lat_slot_;
 }
 
 }  // namespace oe

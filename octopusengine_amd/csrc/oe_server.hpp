@@ -58,6 +58,7 @@ public:
     };
     Stats stats();
     std::vector<double> latencies();
+    std::vector<int> latency_slots();   // slot of each entry of latencies()
 
 private:
     using Clock = std::chrono::steady_clock;
@@ -110,6 +111,7 @@ private:
     uint64_t next_ticket_ = 1;
     int64_t submitted_ = 0, completed_ = 0, vertex_updates_ = 0, batches_ = 0, batch_sum_ = 0, max_batch_seen_ = 0;
     std::vector<double> lat_ms_;
+    std::vector<int> lat_slot_;
     std::string error_;
     std::vector<std::thread> threads_;
 };
