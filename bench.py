@@ -548,6 +548,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--eager-only", action="store_true",
                     help="batched leg: only the eager timed region (no graph, latency or plain loop)")
+    ap.add_argument("--single-only", action="store_true",
+                    help="run only the single-scene leg (P2 64x16x16) and print its figures (profiling)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run a device copy of this many bytes (rocprof counter calibration)")
     args = ap.parse_args()
@@ -558,6 +560,10 @@ def main():
     import octopusengine_amd as oe
 
     eng = oe.Engine(0)
+    if args.single_only:
+        print(json.dumps(_finite_or_null({"plain_single_scene_p2_64x16x16": plain_single_leg(args, eng)})), flush=True)
+        eng.close()
+        return
     bat = batched_leg(args, eng)
     progress(f"batched leg: {bat['ms_per_step']:.2f} ms per step")
     scenes = bat.pop("_scenes")

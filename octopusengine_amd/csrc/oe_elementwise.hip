@@ -24,43 +24,57 @@ inline int grid_for(long long n) { return static_cast<int>((n + kBlock - 1) / kB
 
 __device__ __forceinline__ float4 f4(float x, float y, float z, float w) { return make_float4(x, y, z, w); }
 
-// y = p + ((p - last_p) * (1/dt) + (g + f w) dt) dt ; last_p = p ; p = y ; f = 0
+// y = p + ((p - last_p) / dt + (g + f w) dt) dt ; last_p = p ; p = y ; f = 0,
+// on the compensated state in FP64.
+__device__ __forceinline__ void vbd_predict_one(const PSView& ps, float4* y, int i, float dt, float gx, float gy,
+                                                float gz, float4* prev_it_p) {
+    const float4 p = ps.p[i];
+    const float4 f = ps.f[i];
+    const float w = ps.w[i];
+    const dvec3 P = dunpack(p), LP = dunpack(ps.last_p[i]);
+    if (prev_it_p) prev_it_p[i] = p;
+    const double ddt = dt;
+    const double ax = gx + static_cast<double>(f.x) * w, ay = gy + static_cast<double>(f.y) * w,
+                 az = gz + static_cast<double>(f.z) * w;
+    const float4 Y = dpack(dvec3{P.x + ((P.x - LP.x) / ddt + ax * ddt) * ddt, P.y + ((P.y - LP.y) / ddt + ay * ddt) * ddt,
+                                 P.z + ((P.z - LP.z) / ddt + az * ddt) * ddt});
+    y[i] = Y;
+    ps.last_p[i] = p;
+    ps.p[i] = Y;
+    ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ __launch_bounds__(kBlock) void k_vbd_predict(PSView ps, float4* y, float4* prev_it_p, float dt, float gx,
                                                         float gy, float gz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ps.n) return;
-    const float4 p = ps.p[i];
-    const float4 lp = ps.last_p[i];
-    const float4 f = ps.f[i];
-    const float w = ps.w[i];
-    if (prev_it_p) prev_it_p[i] = p;
-    const float inv_dt = 1.f / dt;
-    const float ax = gx + f.x * w, ay = gy + f.y * w, az = gz + f.z * w;
-    const float yx = p.x + ((p.x - lp.x) * inv_dt + ax * dt) * dt;
-    const float yy = p.y + ((p.y - lp.y) * inv_dt + ay * dt) * dt;
-    const float yz = p.z + ((p.z - lp.z) * inv_dt + az * dt) * dt;
-    const float4 yv = f4(yx, yy, yz, p.w);
-    y[i] = yv;
-    ps.last_p[i] = p;
-    ps.p[i] = yv;
-    ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+    vbd_predict_one(ps, y, i, dt, gx, gy, gz, prev_it_p);
+}
+
+// v = (p - last_p) / dt
+__device__ __forceinline__ void vbd_velocity_one(const PSView& ps, int i, float dt) {
+    const dvec3 P = dunpack(ps.p[i]), LP = dunpack(ps.last_p[i]);
+    const double ddt = dt;
+    ps.v[i] = f4(static_cast<float>((P.x - LP.x) / ddt), static_cast<float>((P.y - LP.y) / ddt),
+                 static_cast<float>((P.z - LP.z) / ddt), 0.f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_vbd_velocity(PSView ps, float dt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ps.n) return;
-    const float4 p = ps.p[i], lp = ps.last_p[i];
-    ps.v[i] = f4((p.x - lp.x) / dt, (p.y - lp.y) / dt, (p.z - lp.z) / dt, 0.f);
+    vbd_velocity_one(ps, i, dt);
 }
 
+// p = prev2 + omega (p - prev2) from the third iteration on; prev2 = prev1, prev1 = p
 __global__ __launch_bounds__(kBlock) void k_chebyshev(PSView ps, float4* prev_it_p, float4* prev_it2_p, int it,
                                                       float omega) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ps.n) return;
     float4 p = ps.p[i];
     if (it >= 2) {
-        const float4 q = prev_it2_p[i];
-        p = f4(q.x + omega * (p.x - q.x), q.y + omega * (p.y - q.y), q.z + omega * (p.z - q.z), p.w);
+        const dvec3 P = dunpack(p), Q = dunpack(prev_it2_p[i]);
+        const double om = omega;
+        p = dpack(dvec3{Q.x + om * (P.x - Q.x), Q.y + om * (P.y - Q.y), Q.z + om * (P.z - Q.z)});
         ps.p[i] = p;
     }
     prev_it2_p[i] = prev_it_p[i];
@@ -106,22 +120,22 @@ __global__ __launch_bounds__(kBlock) void k_pbd_velocity(PSView ps, float dt, fl
 
 // target = offset + com + R (X - com), X = init_p (rest position);  p = target,
 // v = f = 0, mask = 0.  p holds the displacement x - X, so the stored value is
-// offset + (R d - d) with d = X - com (exactly offset when R = I).
-__device__ __forceinline__ float4 fix_displacement(float4 X, const float com[3], const float off[3], const float R[9],
-                                                   float w) {
+// offset + (R d - d) with d = X - com (exactly offset when R = I), with no
+// low-order part (w = 0).
+__device__ __forceinline__ float4 fix_displacement(float4 X, const float com[3], const float off[3], const float R[9]) {
     const float dx = X.x - com[0], dy = X.y - com[1], dz = X.z - com[2];
     // R is column-major; R d = col0 dx + col1 dy + col2 dz
     const float rx = R[0] * dx + R[3] * dy + R[6] * dz;
     const float ry = R[1] * dx + R[4] * dy + R[7] * dz;
     const float rz = R[2] * dx + R[5] * dy + R[8] * dz;
-    return f4(off[0] + (rx - dx), off[1] + (ry - dy), off[2] + (rz - dz), w);
+    return f4(off[0] + (rx - dx), off[1] + (ry - dy), off[2] + (rz - dz), 0.f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fix(PSView ps, const int* ids, int n, FixTransform t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int id = ids[i];
-    ps.p[id] = fix_displacement(ps.init_p[id], t.com, t.offset, t.R, ps.p[id].w);
+    ps.p[id] = fix_displacement(ps.init_p[id], t.com, t.offset, t.R);
     ps.v[id] = f4(0.f, 0.f, 0.f, 0.f);
     ps.f[id] = f4(0.f, 0.f, 0.f, 0.f);
     ps.mask[id] = 0;
@@ -138,12 +152,41 @@ __device__ __forceinline__ bool box_clamp(float4& u, float4 X, const float pmin[
     return cx || cy || cz;
 }
 
+// The compensated form: the absolute position X + (hi + lo) in FP64 against the
+// box; a clamped component becomes bound - X (packed again); written only when
+// a component is clamped.
+__device__ __forceinline__ void box_clamp_comp(const PSView& ps, int i, float4 X, const float pmin[3],
+                                               const float pmax[3]) {
+    const float4 h = ps.p[i];
+    // the low-order part (|lo| <= half an ulp of hi) can only matter within a
+    // relative 1e-4 of a bound: elsewhere the float sum decides alone
+    const float xf[3] = {X.x + h.x, X.y + h.y, X.z + h.z};
+    bool near = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        near = near || xf[k] < pmin[k] + 1e-4f * (1.f + fabsf(pmin[k])) ||
+               xf[k] > pmax[k] - 1e-4f * (1.f + fabsf(pmax[k]));
+    if (!near) return;
+    dvec3 U = dunpack(h);
+    const double x = X.x + U.x, y = X.y + U.y, z = X.z + U.z;
+    const bool cx = x < pmin[0] || x > pmax[0], cy = y < pmin[1] || y > pmax[1], cz = z < pmin[2] || z > pmax[2];
+    if (!(cx || cy || cz)) return;
+    if (cx) U.x = fmin(fmax(x, static_cast<double>(pmin[0])), static_cast<double>(pmax[0])) - X.x;
+    if (cy) U.y = fmin(fmax(y, static_cast<double>(pmin[1])), static_cast<double>(pmax[1])) - X.y;
+    if (cz) U.z = fmin(fmax(z, static_cast<double>(pmin[2])), static_cast<double>(pmax[2])) - X.z;
+    ps.p[i] = dpack(U);
+}
+
 __global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 pmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ps.n) return;
     if (ps.mask[i] == 0) return;
-    float4 u = ps.p[i];
     const float lo[3] = {pmin.x, pmin.y, pmin.z}, hi[3] = {pmax.x, pmax.y, pmax.z};
+    if (ps.comp) {
+        box_clamp_comp(ps, i, ps.init_p[i], lo, hi);
+        return;
+    }
+    float4 u = ps.p[i];
     if (box_clamp(u, ps.init_p[i], lo, hi)) ps.p[i] = u;   // written only when clamped
 }
 
@@ -173,19 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_predict(BatchVArgs b) {
     const SceneDev& sd = vseg_scene(b, i, ok);
     if (!ok) return;
     const PSView ps = sd.ps;
-    const float dt = b.dt;
-    const float4 p = ps.p[i];
-    const float4 lp = ps.last_p[i];
-    const float4 f = ps.f[i];
-    const float w = ps.w[i];
-    const float inv_dt = 1.f / dt;
-    const float ax = sd.g[0] + f.x * w, ay = sd.g[1] + f.y * w, az = sd.g[2] + f.z * w;
-    const float4 yv = f4(p.x + ((p.x - lp.x) * inv_dt + ax * dt) * dt, p.y + ((p.y - lp.y) * inv_dt + ay * dt) * dt,
-                         p.z + ((p.z - lp.z) * inv_dt + az * dt) * dt, p.w);
-    sd.y[i] = yv;
-    ps.last_p[i] = p;
-    ps.p[i] = yv;
-    ps.f[i] = f4(0.f, 0.f, 0.f, 0.f);
+    vbd_predict_one(ps, sd.y, i, b.dt, sd.g[0], sd.g[1], sd.g[2], nullptr);
 }
 
 // fixation then box limit, fused: a fixed vertex is snapped and masked, every
@@ -204,13 +235,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_fix_box(BatchVArgs b) {
     float4 u = gload(static_cast<const float4*>(ps.p), i);
     const float4 X = gload(static_cast<const float4*>(ps.init_p), i);
     if (is_fixed) {
-        gstore(ps.p, i, fix_displacement(X, sd.fix_com, sd.fix_offset, sd.fix_R, u.w));
+        gstore(ps.p, i, fix_displacement(X, sd.fix_com, sd.fix_offset, sd.fix_R));
         gstore(ps.v, i, f4(0.f, 0.f, 0.f, 0.f));
         gstore(ps.f, i, f4(0.f, 0.f, 0.f, 0.f));
         gstore(ps.mask, i, 0);
         return;
     }
     if (!sd.has_box || mask == 0) return;
+    if (ps.comp) {
+        box_clamp_comp(ps, i, X, sd.box_min, sd.box_max);
+        return;
+    }
     if (box_clamp(u, X, sd.box_min, sd.box_max)) gstore(ps.p, i, u);   // written only when clamped
 }
 
@@ -219,8 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_velocity(BatchVArgs b) {
     const SceneDev& sd = vseg_scene(b, i, ok);
     if (!ok) return;
     const PSView ps = sd.ps;
-    const float4 p = ps.p[i], lp = ps.last_p[i];
-    ps.v[i] = f4((p.x - lp.x) / b.dt, (p.y - lp.y) / b.dt, (p.z - lp.z) / b.dt, 0.f);
+    vbd_velocity_one(ps, i, b.dt);
 }
 
 }  // namespace

@@ -118,12 +118,12 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
             int eid, qc, r;
             work_item(own, t, QC, eid, qc, r);
             const size_t tp = static_cast<size_t>(eid) * S;
+            float4 h[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) h[k] = gload(static_cast<const float4*>(P), gload(a.fem.topo, tp + k));
             float xs[S][3];
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const float4 x = gload(static_cast<const float4*>(P), gload(a.fem.topo, tp + k));
-                xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
-            }
+            for (int k = 0; k < S; ++k) rel_node(h[k], h[0], xs[k]);
 #pragma unroll
             for (int qq = 0; qq < QL; ++qq) {
                 const int q = qc * QL + qq;
@@ -153,7 +153,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
     }
     if (a.mode == GATHER_VBD_RESIDUAL) {
         if (lane == 0) {
-            const float4 rv = vbd_residual(acc, P[vid], a.ps.last_p[vid], a.y[vid], a.ps.m[vid], a.damping, a.dt);
+            const float4 rv = vbd_residual(acc, dunpack(gload(static_cast<const float4*>(P), vid)),
+                                           dunpack(gload(static_cast<const float4*>(a.ps.last_p), vid)),
+                                           dunpack(gload(a.y, vid)), a.ps.m[vid], a.damping, a.dt);
             if (a.out3) {
                 a.out3[3 * static_cast<size_t>(vid)] = rv.x;
                 a.out3[3 * static_cast<size_t>(vid) + 1] = rv.y;
@@ -167,18 +169,17 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
     float du[3] = {0.f, 0.f, 0.f};
     if (lane == 0) {
         const float4 pv = gload(static_cast<const float4*>(P), vid);
+        const dvec3 p = dunpack(pv);
         double dx[3];
-        vbd_local_solve(acc, pv, gload(static_cast<const float4*>(a.ps.last_p), vid), gload(a.y, vid),
+        vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(a.ps.last_p), vid)), dunpack(gload(a.y, vid)),
                         gload(a.ps.m, vid), a.damping, a.dt, dx);
-        const float4 out = make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
-                                       pv.z + static_cast<float>(dx[2]), pv.w);
-        if (a.mode == GATHER_VBD_JACOBI)
-            gstore(a.p_new, vid, out);
-        else
-            gstore(a.ps.p, vid, out);
-        du[0] = out.x - pv.x;
-        du[1] = out.y - pv.y;
-        du[2] = out.z - pv.z;
+        const float4 out = dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]});
+        gstore(a.mode == GATHER_VBD_JACOBI ? a.p_new : a.ps.p, vid, out);
+        // the move of the full value (the cached gradients are built from
+        // hi + lo); zero when the solve left the vertex where it was
+        du[0] = static_cast<float>(dx[0]);
+        du[1] = static_cast<float>(dx[1]);
+        du[2] = static_cast<float>(dx[2]);
     }
     if constexpr (CACHED) {
         // broadcast the move from the group's first lane, then update the
@@ -258,24 +259,16 @@ template <int S, int Q>
 __device__ __forceinline__ void ju_refresh_block(const float4* __restrict__ P, const FemView& f, float* ju, int e0,
                                                  int ne) {
     constexpr int EPB = refresh_epb<Q>();
-    __shared__ float sx[EPB * S * 3];
-    for (int t = threadIdx.x; t < ne * S; t += blockDim.x) {
-        const float4 x = P[f.topo[static_cast<size_t>(e0) * S + t]];
-        sx[3 * t] = x.x;
-        sx[3 * t + 1] = x.y;
-        sx[3 * t + 2] = x.z;
-    }
+    __shared__ float4 sx[EPB * S];
+    for (int t = threadIdx.x; t < ne * S; t += blockDim.x) sx[t] = P[f.topo[static_cast<size_t>(e0) * S + t]];
     __syncthreads();
     const int le = threadIdx.x / Q;
     const int q = threadIdx.x - le * Q;
     if (le >= ne) return;
     float xs[S][3];
+    const float4 ref = sx[le * S];
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-        xs[k][0] = sx[(le * S + k) * 3];
-        xs[k][1] = sx[(le * S + k) * 3 + 1];
-        xs[k][2] = sx[(le * S + k) * 3 + 2];
-    }
+    for (int k = 0; k < S; ++k) rel_node(sx[le * S + k], ref, xs[k]);
     float Ju[9];
     jx_from_nodes<S>(xs, f.dN + q * S * 3, Ju);
     float* g = ju + (static_cast<size_t>(e0 + le) * Q + q) * 9;
@@ -395,12 +388,13 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
 #pragma unroll 1
     for (int t = t0; t < n_inc; t += kFusedThreads) {
         const int pk = gload(own.own, first_own + t);
+        float4 h[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            h[k] = gload(static_cast<const float4*>(ps.p), gload(f.topo, static_cast<size_t>(pk >> 5) * S + k));
         float xs[S][3];
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const float4 x = gload(static_cast<const float4*>(ps.p), gload(f.topo, static_cast<size_t>(pk >> 5) * S + k));
-            xs[k][0] = x.x; xs[k][1] = x.y; xs[k][2] = x.z;
-        }
+        for (int k = 0; k < S; ++k) rel_node(h[k], h[0], xs[k]);
         const int eid = pk >> 5, r = pk & 31;
         Acc9 acc;
         elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);
@@ -426,23 +420,7 @@ __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, 
         __syncthreads();
         // phase 2: one thread per slot, partials in the owners order, local solve
         if (t0 >= n_slots) return;
-        const int slot = first_slot + t0;
-        const int vid = gload(own.slot_vid, slot);
-        if (gload(ps.mask, vid) == 0) return;
-        const int2 nbo = gload(own.slot_nbo, slot);
-        Acc9 acc;
-        acc_zero(acc);
-        for (int j = nbo.y - first_own, e = j + nbo.x; j < e; ++j) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
-        }
-        const float4 pv = gload(static_cast<const float4*>(ps.p), vid);
-        double dx[3];
-        vbd_local_solve(acc, pv, gload(static_cast<const float4*>(ps.last_p), vid), gload(y, vid), gload(ps.m, vid),
-                        damping, dt, dx);
-        gstore(ps.p, vid,
-               make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
-                           pv.z + static_cast<float>(dx[2]), pv.w));
+        solve_slot_direct(ps, own, y, damping, dt, first_slot + t0, part, first_own);
     }
 }
 

@@ -75,20 +75,49 @@ GPU_ParticleSystem::GPU_ParticleSystem(Engine* e, const std::vector<float>& xyz,
 }
 
 PSView GPU_ParticleSystem::get_parameters() const {
-    return PSView{n, p.get(), last_p.get(), init_p.get(), v.get(), f.get(), m.get(), w.get(), mask.get()};
+    return PSView{n,       p.get(),       last_p.get(), init_p.get(), v.get(), f.get(), m.get(), w.get(), mask.get(),
+                  comp ? 1 : 0};
+}
+
+// the records' w lanes are zero until a compensated integrator writes them
+void GPU_ParticleSystem::enable_compensation() { comp = true; }
+
+std::vector<double> GPU_ParticleSystem::full(const DBuf<float4>& rec) const {
+    std::vector<float4> h;
+    rec.download(h, eng->stream);
+    std::vector<double> u(3 * h.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+        const dvec3 d = comp ? dunpack(h[i]) : dvec3{h[i].x, h[i].y, h[i].z};
+        u[3 * i] = d.x;
+        u[3 * i + 1] = d.y;
+        u[3 * i + 2] = d.z;
+    }
+    return u;
 }
 
 void GPU_ParticleSystem::get_position(std::vector<float>& xyz) const {
-    std::vector<float4> h;
-    p.download(h, eng->stream);
-    from_f4(h, xyz);
-    for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = rest[i] + xyz[i];
+    const std::vector<double> u = full(p);
+    xyz.resize(u.size());
+    for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = static_cast<float>(rest[i] + u[i]);
 }
 void GPU_ParticleSystem::get_prev_position(std::vector<float>& xyz) const {
-    std::vector<float4> h;
-    last_p.download(h, eng->stream);
-    from_f4(h, xyz);
-    for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = rest[i] + xyz[i];
+    const std::vector<double> u = full(last_p);
+    xyz.resize(u.size());
+    for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = static_cast<float>(rest[i] + u[i]);
+}
+
+// xyz (absolute, 3n floats) -> displacement records from the double
+// difference: packed with its low-order part when the state is compensated,
+// rounded to float with w = 0 otherwise
+void GPU_ParticleSystem::upload_split(const float* xyz, DBuf<float4>& rec) {
+    std::vector<float4> h(n);
+    for (int i = 0; i < n; ++i) {
+        const dvec3 u{static_cast<double>(xyz[3 * i]) - rest[3 * i], static_cast<double>(xyz[3 * i + 1]) - rest[3 * i + 1],
+                      static_cast<double>(xyz[3 * i + 2]) - rest[3 * i + 2]};
+        h[i] = comp ? dpack(u)
+                    : make_float4(static_cast<float>(u.x), static_cast<float>(u.y), static_cast<float>(u.z), 0.f);
+    }
+    rec.upload(h, eng->stream);
 }
 void GPU_ParticleSystem::get_init_position(std::vector<float>& xyz) const {
     std::vector<float4> h;
@@ -113,13 +142,11 @@ std::vector<float> GPU_ParticleSystem::to_displacement(const float* xyz) const {
     for (size_t i = 0; i < u.size(); ++i) u[i] = static_cast<float>(static_cast<double>(xyz[i]) - rest[i]);
     return u;
 }
-void GPU_ParticleSystem::set_position(const std::vector<float>& xyz) {
-    p.upload(to_f4(to_displacement(xyz.data())), eng->stream);
-}
+void GPU_ParticleSystem::set_position(const std::vector<float>& xyz) { upload_split(xyz.data(), p); }
 void GPU_ParticleSystem::set_velocity(const std::vector<float>& xyz) { v.upload(to_f4(xyz), eng->stream); }
 void GPU_ParticleSystem::set_state(const float* pp, const float* lp, const float* vv) {
-    if (pp) p.upload(to_f4(to_displacement(pp)), eng->stream);
-    if (lp) last_p.upload(to_f4(to_displacement(lp)), eng->stream);
+    if (pp) upload_split(pp, p);
+    if (lp) upload_split(lp, last_p);
     if (vv) v.upload(to_f4(std::vector<float>(vv, vv + 3 * static_cast<size_t>(n))), eng->stream);
 }
 size_t GPU_ParticleSystem::device_bytes() const {
@@ -150,6 +177,7 @@ GPU_VBD::GPU_VBD(GPU_ParticleSystem* ps, int it, float rho) : iteration(it), _rh
     if (it < 0) throw std::invalid_argument("iteration count must be >= 0");
     MemoryPool* pool = &ps->eng->pool;
     stream = ps->eng->stream;
+    ps->enable_compensation();   // the VBD state keeps low-order parts in the records' w lanes
     y.allocate(pool, ps->n);
     OE_HIP(hipMemcpyAsync(y.get(), ps->p.get(), sizeof(float4) * ps->n, hipMemcpyDeviceToDevice, stream));
     if (rho > 0.f) {
@@ -203,6 +231,8 @@ GPU_FEM::GPU_FEM(Engine* e, Element el, const HostMesh& mesh, float young, float
     S = fc.S;
     Q = fc.Q;
     E = fc.E;
+    if (static_cast<uint64_t>(E) * static_cast<uint64_t>(Q) >= (uint64_t{1} << 32))
+        throw std::length_error("mesh too large: elements x quadrature points must stay below 2^32");
     mt.material = material;
     lame_parameters(young, poisson, &mt.lambda, &mt.mu);
     mt.alpha = (material == Stable_NeoHooke) ? 1.f + mt.mu / mt.lambda : 1.f;

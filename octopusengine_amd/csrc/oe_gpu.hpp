@@ -60,6 +60,10 @@ struct GPU_ParticleSystem {
     void set_state(const float* p, const float* last_p, const float* v);
     size_t device_bytes() const;
     std::vector<float> to_displacement(const float* xyz) const;
+    // Keep low-order parts of the displacement records in their w lanes (see
+    // dpack): set by the VBD integrator, whose state is then hi + lo per vertex.
+    void enable_compensation();
+    bool compensated() const { return comp; }
 
     Engine* eng;
     int n;
@@ -67,6 +71,12 @@ struct GPU_ParticleSystem {
     DBuf<float4> p, last_p, init_p, v, f;
     DBuf<float> m, w;
     DBuf<int> mask;
+
+private:
+    bool comp = false;
+    // displacement of every vertex, hi + lo in double (lo = 0 without compensation)
+    std::vector<double> full(const DBuf<float4>& rec) const;
+    void upload_split(const float* xyz, DBuf<float4>& rec);
 };
 
 struct GPU_Dynamic {

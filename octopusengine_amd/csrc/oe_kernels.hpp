@@ -176,21 +176,20 @@ __device__ __forceinline__ void nh_contrib(const float Jx[9], const float JXi[9]
 // Residual force of the local VBD problem at the current position
 // (reference kernel_vbd_compute_residual, src/GPU/VBD/GPU_VBD_FEM.cu:482-542):
 //   r = f - (k/dt) H (p - last_p) - (m/dt^2)(p - y)
-__device__ __forceinline__ float4 vbd_residual(const Acc9& a, float4 p, float4 lp, float4 y, float m, float damping,
-                                               float dt) {
+__device__ __forceinline__ float4 vbd_residual(const Acc9& a, const dvec3& p, const dvec3& lp, const dvec3& y, float m,
+                                               float damping, float dt) {
     double r[3] = {a.v[0], a.v[1], a.v[2]};
     if (damping != 0.f) {
         const double k = static_cast<double>(damping) / dt;
-        const double u0 = static_cast<double>(p.x) - lp.x, u1 = static_cast<double>(p.y) - lp.y,
-                     u2 = static_cast<double>(p.z) - lp.z;
+        const double u0 = p.x - lp.x, u1 = p.y - lp.y, u2 = p.z - lp.z;
         r[0] -= k * (a.v[3] * u0 + a.v[4] * u1 + a.v[5] * u2);
         r[1] -= k * (a.v[4] * u0 + a.v[6] * u1 + a.v[7] * u2);
         r[2] -= k * (a.v[5] * u0 + a.v[7] * u1 + a.v[8] * u2);
     }
     const double mh2 = static_cast<double>(m) / (static_cast<double>(dt) * dt);
-    r[0] -= mh2 * (static_cast<double>(p.x) - y.x);
-    r[1] -= mh2 * (static_cast<double>(p.y) - y.y);
-    r[2] -= mh2 * (static_cast<double>(p.z) - y.z);
+    r[0] -= mh2 * (p.x - y.x);
+    r[1] -= mh2 * (p.y - y.y);
+    r[2] -= mh2 * (p.z - y.z);
     return make_float4(static_cast<float>(r[0]), static_cast<float>(r[1]), static_cast<float>(r[2]), 0.f);
 }
 
@@ -199,15 +198,14 @@ __device__ __forceinline__ float4 vbd_residual(const Acc9& a, float4 p, float4 l
 //   f -= (k/dt) H (p - last_p);  H += (k/dt) H
 //   f -= (m/dt^2)(p - y);        H += (m/dt^2) I
 //   dx = |det H| > 1e-6 ? H^-1 f : 0
-__device__ __forceinline__ void vbd_local_solve(const Acc9& a, float4 p, float4 lp, float4 y, float m, float damping,
-                                                float dt, double dx[3]) {
+__device__ __forceinline__ void vbd_local_solve(const Acc9& a, const dvec3& p, const dvec3& lp, const dvec3& y, float m,
+                                                float damping, float dt, double dx[3]) {
     double f0 = a.v[0], f1 = a.v[1], f2 = a.v[2];
     double h00 = a.v[3], h10 = a.v[4], h20 = a.v[5], h11 = a.v[6], h21 = a.v[7], h22 = a.v[8];
     const double ddt = dt;
     if (damping > 0.f) {
         const double k = static_cast<double>(damping) / ddt;
-        const double u0 = static_cast<double>(p.x) - lp.x, u1 = static_cast<double>(p.y) - lp.y,
-                     u2 = static_cast<double>(p.z) - lp.z;
+        const double u0 = p.x - lp.x, u1 = p.y - lp.y, u2 = p.z - lp.z;
         f0 -= k * (h00 * u0 + h10 * u1 + h20 * u2);
         f1 -= k * (h10 * u0 + h11 * u1 + h21 * u2);
         f2 -= k * (h20 * u0 + h21 * u1 + h22 * u2);
@@ -215,9 +213,9 @@ __device__ __forceinline__ void vbd_local_solve(const Acc9& a, float4 p, float4 
         h00 *= s; h10 *= s; h20 *= s; h11 *= s; h21 *= s; h22 *= s;
     }
     const double mh2 = static_cast<double>(m) / (ddt * ddt);
-    f0 -= mh2 * (static_cast<double>(p.x) - y.x);
-    f1 -= mh2 * (static_cast<double>(p.y) - y.y);
-    f2 -= mh2 * (static_cast<double>(p.z) - y.z);
+    f0 -= mh2 * (p.x - y.x);
+    f1 -= mh2 * (p.y - y.y);
+    f2 -= mh2 * (p.z - y.z);
     h00 += mh2; h11 += mh2; h22 += mh2;
     const double c00 = h11 * h22 - h21 * h21;
     const double c01 = h21 * h20 - h10 * h22;
@@ -247,10 +245,30 @@ __device__ __forceinline__ void solve_slot_from_lds(const PSView& ps, const Slot
         for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
     }
     double dx[3];
-    vbd_local_solve(acc, st.p, st.last_p, st.y, st.m, damping, dt, dx);
-    gstore(ps.p, st.vid,
-           make_float4(st.p.x + static_cast<float>(dx[0]), st.p.y + static_cast<float>(dx[1]),
-                       st.p.z + static_cast<float>(dx[2]), st.p.w));
+    const dvec3 p = dunpack(st.p);
+    vbd_local_solve(acc, p, dunpack(st.last_p), dunpack(st.y), st.m, damping, dt, dx);
+    gstore(ps.p, st.vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}));
+}
+
+// Phase 2 of the fused task kernels for one slot: add the slot's partials
+// (part[j], j = its owners index - first_own, in the owners order) and solve.
+__device__ __forceinline__ void solve_slot_direct(const PSView& ps, const OwnersView& own, const float4* y,
+                                                  float damping, float dt, int slot, const float (*part)[9],
+                                                  int first_own) {
+    const int vid = gload(own.slot_vid, slot);
+    if (gload(ps.mask, vid) == 0) return;
+    const int2 nbo = gload(own.slot_nbo, slot);
+    Acc9 acc;
+    acc_zero(acc);
+    for (int j = nbo.y - first_own, e = j + nbo.x; j < e; ++j) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
+    }
+    const dvec3 p = dunpack(gload(static_cast<const float4*>(ps.p), vid));
+    double dx[3];
+    vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(ps.last_p), vid)), dunpack(gload(y, vid)),
+                    gload(ps.m, vid), damping, dt, dx);
+    gstore(ps.p, vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}));
 }
 
 }  // namespace oe

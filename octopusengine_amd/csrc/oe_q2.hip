@@ -146,8 +146,23 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
     if (P.l >= 27) V = 0.f;
     const float dnr[3] = {gload(f.dN, (P.q * 27 + r) * 3), gload(f.dN, (P.q * 27 + r) * 3 + 1),
                           gload(f.dN, (P.q * 27 + r) * 3 + 2)};
-    // stage 0
-    sh.u[P.place] = u;
+    // stage 0: the node's value relative to node 0 of the element (lane 0 of
+    // the half, read with v_readlane: a half that has left the task loop is
+    // read but not used)
+    {
+        const bool upper = (threadIdx.x & 32) != 0;
+        const float4 ref = make_float4(
+            upper ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.x), 32))
+                  : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.x), 0)),
+            upper ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.y), 32))
+                  : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.y), 0)),
+            upper ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.z), 32))
+                  : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.z), 0)),
+            0.f);
+        float d[3];
+        rel_node(u, ref, d);
+        sh.u[P.place] = make_float4(d[0], d[1], d[2], 0.f);
+    }
     half_sync();
     // stage 1: lane (a, j, k); u_abc sits at row a * 9 + b * 3 + c
     {
@@ -236,28 +251,15 @@ __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, 
 }
 
 // Phase 2: thread t < slot count adds the partials of slot first_slot + t in
-// the owners order and solves.
+// the owners order and solves.  t is rebuilt from the wave's first thread (a
+// scalar) and the lane count, which the compiler recomputes where threadIdx.x
+// would otherwise be kept in scratch across the task loop.
 __device__ __forceinline__ void q2_solve_slots(const PSView& ps, const OwnersView& own, const float4* y,
-                                              float damping, float dt, const int4 blk, const float (*part)[9]) {
-    const int t = threadIdx.x;
+                                              float damping, float dt, const int4 blk, const float (*part)[9],
+                                              int wave_first) {
+    const int t = wave_first + static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
     if (t >= blk.y) return;
-    const int slot = blk.x + t;
-    const int vid = gload(own.slot_vid, slot);
-    if (gload(ps.mask, vid) == 0) return;
-    const int2 nbo = gload(own.slot_nbo, slot);
-    Acc9 acc;
-    acc_zero(acc);
-    for (int j = nbo.y - blk.z, e = j + nbo.x; j < e; ++j) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
-    }
-    const float4 pv = gload(static_cast<const float4*>(ps.p), vid);
-    double dx[3];
-    vbd_local_solve(acc, pv, gload(static_cast<const float4*>(ps.last_p), vid), gload(y, vid), gload(ps.m, vid),
-                    damping, dt, dx);
-    gstore(ps.p, vid,
-           make_float4(pv.x + static_cast<float>(dx[0]), pv.y + static_cast<float>(dx[1]),
-                       pv.z + static_cast<float>(dx[2]), pv.w));
+    solve_slot_direct(ps, own, y, damping, dt, blk.x + t, part, blk.z);
 }
 
 template <bool LAT>
@@ -276,10 +278,11 @@ __device__ __forceinline__ void q2_block(const PSView& ps, const FemView& fem, c
         if (static_cast<int>(threadIdx.x) < blk.y && st.free != 0)
             solve_slot_from_lds(ps, st, part, blk.z, damping, dt);
     } else {
+        const int wave_first = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) & ~63);
         const LanePlan P = lane_plan(basis);
         q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
         __syncthreads();
-        q2_solve_slots(ps, own, y, damping, dt, blk, part);
+        q2_solve_slots(ps, own, y, damping, dt, blk, part, wave_first);
     }
 }
 
@@ -289,10 +292,11 @@ __global__ __launch_bounds__(kQ2FusedThreads, LAT ? 1 : 6) void k_q2_fused(Q2Fus
     q2_block<LAT>(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
 }
 
-// The throughput form is held to 80 VGPRs (6 waves per SIMD).  The compiler
-// then keeps 2 dwords in scratch, and the 256-scene batch still steps 3%
-// faster than at 5 waves, where the kernel needs 82 VGPRs and spills nothing
-// (50.6 against 52.2 ms per step).  The latency form keeps its registers.
+// The throughput form is held to 80 VGPRs (6 waves per SIMD), which it fits
+// without scratch: the element-record index is formed in 32 bits and the
+// phase-2 thread index is rebuilt from a scalar (q2_solve_slots).  At 5 waves
+// the 256-scene batch stepped 3% slower.  The latency form keeps its
+// registers.
 template <bool LAT>
 __global__ __launch_bounds__(kQ2FusedThreads, LAT ? 1 : 6) void k_batch_q2_fused(BatchQ2FusedArgs b) {
     const int bid = xcd_block(blockIdx.x, gridDim.x);

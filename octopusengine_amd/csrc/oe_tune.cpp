@@ -32,35 +32,35 @@ void cache_store(const TuneRecord& r) {
     g_cache[{r.device, r.element, r.bucket, r.form}] = r;
 }
 
-// p, last_p, v and f of a set of particle systems, copied on the device.
+// The state a timing run moves, copied on the device: p, last_p, v, f (with
+// the low-order parts in the w lanes) and the VBD inertia target y of every
+// scene.
 class Snapshot {
 public:
-    Snapshot(Engine* eng, const std::vector<GPU_ParticleSystem*>& systems) : eng_(eng), ps_(systems) {
-        for (GPU_ParticleSystem* ps : ps_) {
-            for (DBuf<float4>* src : fields(ps)) {
-                DBuf<float4> copy(&eng_->pool, static_cast<size_t>(ps->n));
-                OE_HIP(hipMemcpyAsync(copy.get(), src->get(), sizeof(float4) * ps->n, hipMemcpyDeviceToDevice,
-                                      eng_->stream));
-                saved_.push_back(std::move(copy));
-            }
+    Snapshot(Engine* eng, const std::vector<Scene*>& scenes) : eng_(eng) {
+        for (Scene* sc : scenes) {
+            GPU_ParticleSystem* ps = sc->ps.get();
+            for (DBuf<float4>* b : {&ps->p, &ps->last_p, &ps->v, &ps->f})
+                if (b->get()) fields_.push_back(b);
+            if (auto* vbd = dynamic_cast<GPU_VBD*>(sc->integrator.get())) fields_.push_back(&vbd->y);
+        }
+        for (DBuf<float4>* src : fields_) {
+            DBuf<float4> copy(&eng_->pool, src->size());
+            OE_HIP(hipMemcpyAsync(copy.get(), src->get(), src->bytes(), hipMemcpyDeviceToDevice, eng_->stream));
+            saved_.push_back(std::move(copy));
         }
         OE_HIP(hipStreamSynchronize(eng_->stream));
     }
     void restore() {
-        size_t k = 0;
-        for (GPU_ParticleSystem* ps : ps_)
-            for (DBuf<float4>* dst : fields(ps)) {
-                OE_HIP(hipMemcpyAsync(dst->get(), saved_[k].get(), sizeof(float4) * ps->n, hipMemcpyDeviceToDevice,
-                                      eng_->stream));
-                ++k;
-            }
+        for (size_t k = 0; k < fields_.size(); ++k)
+            OE_HIP(hipMemcpyAsync(fields_[k]->get(), saved_[k].get(), fields_[k]->bytes(), hipMemcpyDeviceToDevice,
+                                  eng_->stream));
         OE_HIP(hipStreamSynchronize(eng_->stream));
     }
 
 private:
-    static std::vector<DBuf<float4>*> fields(GPU_ParticleSystem* ps) { return {&ps->p, &ps->last_p, &ps->v, &ps->f}; }
     Engine* eng_;
-    std::vector<GPU_ParticleSystem*> ps_;
+    std::vector<DBuf<float4>*> fields_;
     std::vector<DBuf<float4>> saved_;
 };
 
@@ -205,7 +205,7 @@ TuneRecord autotune_scene(Scene& sc, bool force) {
     const bool vbd = sc.vbd_fem != nullptr;
     std::vector<GatherTile> cands = tile_candidates(sc.mesh.element, vbd, vbd && tasks_possible(sc));
     const GatherTile before = vbd ? sc.vbd_fem->tile : sc.explicit_fem->tile;
-    Snapshot snap(sc.eng, {sc.ps.get()});
+    Snapshot snap(sc.eng, {&sc});
     Events ev;
     std::vector<std::pair<int, std::vector<bool>>> seen;
     std::vector<GatherTile> unique;
@@ -263,7 +263,6 @@ TuneRecord autotune_scene(Scene& sc, bool force) {
 std::vector<TuneRecord> autotune_batch(Batch& b, bool force) {
     std::map<int, int64_t> vertices;
     std::map<int, bool> tasks_ok;
-    std::vector<GPU_ParticleSystem*> systems;
     std::vector<int64_t> steps_before;
     for (Scene* s : b.scenes) {
         if (!s) throw std::logic_error("a scene of this batch was destroyed");
@@ -271,7 +270,6 @@ std::vector<TuneRecord> autotune_batch(Batch& b, bool force) {
         const auto it = tasks_ok.find(s->mesh.element);
         const bool ok = tasks_possible(*s);
         tasks_ok[s->mesh.element] = it == tasks_ok.end() ? ok : (it->second && ok);
-        systems.push_back(s->ps.get());
         steps_before.push_back(s->steps_done);
     }
     const std::map<int, GatherTile> tiles_before = b.tiles;
@@ -316,7 +314,7 @@ std::vector<TuneRecord> autotune_batch(Batch& b, bool force) {
         b.timer.tag_ms = timer_tag_ms;
         b.timer.tag_launches = timer_tag_launches;
     };
-    Snapshot snap(b.eng, systems);
+    Snapshot snap(b.eng, b.scenes);
     b.timer.enabled = true;
     try {
         for (int el : to_time) {

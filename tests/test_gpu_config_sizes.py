@@ -63,10 +63,10 @@ def test_vbd_p2_64x16x16_one_step(engine):
     assert err <= TOL1
 
 
-@pytest.mark.xfail(reason="measured 3.6e-3 m after 100 steps (bound 1.06e-3 m): the FP64 oracle with its "
-                          "displacements rounded to float after every update ends 5.1e-3 m from itself, so FP32 "
-                          "position storage alone exceeds the bound on this scene", strict=False)
 def test_vbd_p2_64x16x16_100_steps_against_oracle_fixture(engine):
+    """100 steps hold the bound only with the compensated displacement state:
+    the FP64 oracle with its displacements rounded to float after every update
+    ends 5.1e-3 m from itself on this scene, above the 1.06e-3 m bound."""
     import octopusengine_amd as oe
 
     fx = np.load(FIXTURE)
