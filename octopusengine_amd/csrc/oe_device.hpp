@@ -144,58 +144,51 @@ struct PSView {
     // 1 when the displacement records carry low-order parts in w (the VBD
     // state, see dpack); 0 when w is zero and the state is plain FP32.
     int comp;
+    // quantum of the low-order parts (a power of two) and its inverse
+    float lo_unit, lo_inv;
 };
 
 // Compensated displacement records.  The VBD integrator keeps p, last_p, its
-// inertia target y and its Chebyshev history to about 34 significant bits at
-// no extra memory traffic: a record holds hi = float(u) in x, y, z and the
-// remainder u - hi, rounded to a multiple of 2^-10 ulp(hi), as three 10-bit
-// two's-complement integers in the bits of w (x in bits 0-9, y in 10-19, z in
-// 20-29).  |u - hi| <= ulp(hi)/2 makes |q| <= 512; q is stored clamped to
-// +-511.  Bits 30-31 stay zero, so w always reads as a finite non-negative
-// float, and w = 0 is the plain float value.  The element kernels read x, y,
-// z alone (FP32); the predictor, the constraints and the per-vertex solve
-// work on hi + lo in FP64 and store the result packed again, so that the
-// state is not rounded to float at every update (FP32 displacements of order
-// 1 m carry 1e-7 m of rounding per update, which a soft beam amplifies over a
-// hundred steps).
+// inertia target y and its Chebyshev history to ~34 bits over the scene's
+// range at no extra memory traffic: a record holds hi = float(u) in x, y, z
+// and the remainder u - hi, rounded to a multiple of the scene's quantum D,
+// as three 10-bit two's-complement integers in the bits of w (x in bits 0-9,
+// y in 10-19, z in 20-29).  D = U 2^-34, U the power of two at or above 8
+// times the scene's rest bounding-box diagonal (lo_unit_for): for |u| < U,
+// |u - hi| <= U 2^-25 = 512 D, stored clamped to +-511 D; larger
+// displacements keep the clamped part, and their precision falls back toward
+// float's.  Bits 30-31 stay zero, so w always reads as a finite non-negative
+// float, and w = 0 is the plain float value.  The element kernels read the
+// node values as (hi - ref) + lo in FP32 (rel_node); the predictor, the
+// constraints and the per-vertex solve work on hi + lo in FP64 and store the
+// result packed again, so that the state is not rounded to float at every
+// update.
 struct dvec3 {
     double x, y, z;
 };
-// log2 of the unit of the low-order part of hi: 2^-10 ulp(hi) = 2^(e - 160)
-// for the biased exponent e of hi (the subnormal spacing 2^-149 for e = 0)
-__host__ __device__ __forceinline__ int lo_exp(float h) {
-    const unsigned e = (__builtin_bit_cast(unsigned, h) >> 23) & 0xffu;
-    return static_cast<int>(e ? e : 1u) - 160;
-}
-// 2^k as a double, for -1022 <= k <= 1023
-__host__ __device__ __forceinline__ double pow2(int k) {
-    return __builtin_bit_cast(double, static_cast<unsigned long long>(k + 1023) << 52);
-}
-__host__ __device__ __forceinline__ unsigned lo_code(double r, float h) {
-    const double q = fmin(fmax(rint(r * pow2(-lo_exp(h))), -511.0), 511.0);
+__host__ __device__ __forceinline__ unsigned lo_code(double r, double inv) {
+    const double q = fmin(fmax(rint(r * inv), -511.0), 511.0);
     return static_cast<unsigned>(static_cast<int>(q)) & 0x3ffu;
 }
-__host__ __device__ __forceinline__ double lo_value(unsigned w, int shift, float h) {
-    const int q = static_cast<int>(w << (22 - shift)) >> 22;
-    return q * pow2(lo_exp(h));
+// the 10-bit field of w at bit `shift`, sign-extended
+__host__ __device__ __forceinline__ int lo_int(unsigned w, int shift) {
+    return static_cast<int>(w << (22 - shift)) >> 22;
 }
-__host__ __device__ __forceinline__ float4 dpack(const dvec3& u) {
+__host__ __device__ __forceinline__ float4 dpack(const dvec3& u, float inv_unit) {
+    const double inv = inv_unit;
     const float hx = static_cast<float>(u.x), hy = static_cast<float>(u.y), hz = static_cast<float>(u.z);
-    const unsigned w = lo_code(u.x - hx, hx) | lo_code(u.y - hy, hy) << 10 | lo_code(u.z - hz, hz) << 20;
+    const unsigned w = lo_code(u.x - hx, inv) | lo_code(u.y - hy, inv) << 10 | lo_code(u.z - hz, inv) << 20;
     return make_float4(hx, hy, hz, __builtin_bit_cast(float, w));
 }
-__host__ __device__ __forceinline__ dvec3 dunpack(float4 h) {
+__host__ __device__ __forceinline__ dvec3 dunpack(float4 h, float unit) {
     const unsigned w = __builtin_bit_cast(unsigned, h.w);
-    return dvec3{h.x + lo_value(w, 0, h.x), h.y + lo_value(w, 10, h.y), h.z + lo_value(w, 20, h.z)};
+    const double d = unit;
+    return dvec3{h.x + lo_int(w, 0) * d, h.y + lo_int(w, 10) * d, h.z + lo_int(w, 20) * d};
 }
-
-// The low-order part of one component of a displacement record as a float
-// (q * 2^-10 ulp(hi), see dpack); shift is 0, 10 or 20 for x, y, z.
-__device__ __forceinline__ float lo_f32(float h, unsigned w, int shift) {
-    const int q = static_cast<int>(w << (22 - shift)) >> 22;
-    const int e = static_cast<int>((__float_as_uint(h) >> 23) & 0xffu);
-    return ldexpf(static_cast<float>(q), e - 160);
+// D for a scene whose rest bounding box has diagonal `diag` (metres)
+inline float lo_unit_for(double diag) {
+    const double L = fmin(fmax(diag, 1e-6), 1e9);
+    return static_cast<float>(ldexp(1.0, static_cast<int>(ceil(log2(8.0 * L))) - 34));
 }
 // A node's value for an element's deformation gradient: (hi - ref) + lo in
 // FP32, where ref is the high part of a displacement common to the element's
@@ -205,11 +198,11 @@ __device__ __forceinline__ float lo_f32(float h, unsigned w, int shift) {
 // sum over the nodes of the large values themselves would cancel in FP32, and
 // the low-order part restores the bits the high part rounded away.  On
 // records without low-order parts (w = 0) it is hi - ref.
-__device__ __forceinline__ void rel_node(float4 u, float4 ref, float out[3]) {
+__device__ __forceinline__ void rel_node(float4 u, float4 ref, float unit, float out[3]) {
     const unsigned w = __float_as_uint(u.w);
-    out[0] = (u.x - ref.x) + lo_f32(u.x, w, 0);
-    out[1] = (u.y - ref.y) + lo_f32(u.y, w, 10);
-    out[2] = (u.z - ref.z) + lo_f32(u.z, w, 20);
+    out[0] = fmaf(static_cast<float>(lo_int(w, 0)), unit, u.x - ref.x);
+    out[1] = fmaf(static_cast<float>(lo_int(w, 10)), unit, u.y - ref.y);
+    out[2] = fmaf(static_cast<float>(lo_int(w, 20)), unit, u.z - ref.z);
 }
 
 // Per-mesh FEM data (reference GPU_FEM_Pameters, include/GPU/GPU_FEM.h:49-59).

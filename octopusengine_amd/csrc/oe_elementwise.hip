@@ -31,13 +31,14 @@ __device__ __forceinline__ void vbd_predict_one(const PSView& ps, float4* y, int
     const float4 p = ps.p[i];
     const float4 f = ps.f[i];
     const float w = ps.w[i];
-    const dvec3 P = dunpack(p), LP = dunpack(ps.last_p[i]);
+    const dvec3 P = dunpack(p, ps.lo_unit), LP = dunpack(ps.last_p[i], ps.lo_unit);
     if (prev_it_p) prev_it_p[i] = p;
     const double ddt = dt;
     const double ax = gx + static_cast<double>(f.x) * w, ay = gy + static_cast<double>(f.y) * w,
                  az = gz + static_cast<double>(f.z) * w;
     const float4 Y = dpack(dvec3{P.x + ((P.x - LP.x) / ddt + ax * ddt) * ddt, P.y + ((P.y - LP.y) / ddt + ay * ddt) * ddt,
-                                 P.z + ((P.z - LP.z) / ddt + az * ddt) * ddt});
+                                 P.z + ((P.z - LP.z) / ddt + az * ddt) * ddt},
+                           ps.lo_inv);
     y[i] = Y;
     ps.last_p[i] = p;
     ps.p[i] = Y;
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_vbd_predict(PSView ps, float4* y, fl
 
 // v = (p - last_p) / dt
 __device__ __forceinline__ void vbd_velocity_one(const PSView& ps, int i, float dt) {
-    const dvec3 P = dunpack(ps.p[i]), LP = dunpack(ps.last_p[i]);
+    const dvec3 P = dunpack(ps.p[i], ps.lo_unit), LP = dunpack(ps.last_p[i], ps.lo_unit);
     const double ddt = dt;
     ps.v[i] = f4(static_cast<float>((P.x - LP.x) / ddt), static_cast<float>((P.y - LP.y) / ddt),
                  static_cast<float>((P.z - LP.z) / ddt), 0.f);
@@ -72,9 +73,9 @@ __global__ __launch_bounds__(kBlock) void k_chebyshev(PSView ps, float4* prev_it
     if (i >= ps.n) return;
     float4 p = ps.p[i];
     if (it >= 2) {
-        const dvec3 P = dunpack(p), Q = dunpack(prev_it2_p[i]);
+        const dvec3 P = dunpack(p, ps.lo_unit), Q = dunpack(prev_it2_p[i], ps.lo_unit);
         const double om = omega;
-        p = dpack(dvec3{Q.x + om * (P.x - Q.x), Q.y + om * (P.y - Q.y), Q.z + om * (P.z - Q.z)});
+        p = dpack(dvec3{Q.x + om * (P.x - Q.x), Q.y + om * (P.y - Q.y), Q.z + om * (P.z - Q.z)}, ps.lo_inv);
         ps.p[i] = p;
     }
     prev_it2_p[i] = prev_it_p[i];
@@ -167,14 +168,14 @@ __device__ __forceinline__ void box_clamp_comp(const PSView& ps, int i, float4 X
         near = near || xf[k] < pmin[k] + 1e-4f * (1.f + fabsf(pmin[k])) ||
                xf[k] > pmax[k] - 1e-4f * (1.f + fabsf(pmax[k]));
     if (!near) return;
-    dvec3 U = dunpack(h);
+    dvec3 U = dunpack(h, ps.lo_unit);
     const double x = X.x + U.x, y = X.y + U.y, z = X.z + U.z;
     const bool cx = x < pmin[0] || x > pmax[0], cy = y < pmin[1] || y > pmax[1], cz = z < pmin[2] || z > pmax[2];
     if (!(cx || cy || cz)) return;
     if (cx) U.x = fmin(fmax(x, static_cast<double>(pmin[0])), static_cast<double>(pmax[0])) - X.x;
     if (cy) U.y = fmin(fmax(y, static_cast<double>(pmin[1])), static_cast<double>(pmax[1])) - X.y;
     if (cz) U.z = fmin(fmax(z, static_cast<double>(pmin[2])), static_cast<double>(pmax[2])) - X.z;
-    ps.p[i] = dpack(U);
+    ps.p[i] = dpack(U, ps.lo_inv);
 }
 
 __global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 pmax) {

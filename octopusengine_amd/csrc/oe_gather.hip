@@ -123,7 +123,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
             for (int k = 0; k < S; ++k) h[k] = gload(static_cast<const float4*>(P), gload(a.fem.topo, tp + k));
             float xs[S][3];
 #pragma unroll
-            for (int k = 0; k < S; ++k) rel_node(h[k], h[0], xs[k]);
+            for (int k = 0; k < S; ++k) rel_node(h[k], h[0], a.ps.lo_unit, xs[k]);
 #pragma unroll
             for (int qq = 0; qq < QL; ++qq) {
                 const int q = qc * QL + qq;
@@ -153,9 +153,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
     }
     if (a.mode == GATHER_VBD_RESIDUAL) {
         if (lane == 0) {
-            const float4 rv = vbd_residual(acc, dunpack(gload(static_cast<const float4*>(P), vid)),
-                                           dunpack(gload(static_cast<const float4*>(a.ps.last_p), vid)),
-                                           dunpack(gload(a.y, vid)), a.ps.m[vid], a.damping, a.dt);
+            const float u = a.ps.lo_unit;
+            const float4 rv = vbd_residual(acc, dunpack(gload(static_cast<const float4*>(P), vid), u),
+                                           dunpack(gload(static_cast<const float4*>(a.ps.last_p), vid), u),
+                                           dunpack(gload(a.y, vid), u), a.ps.m[vid], a.damping, a.dt);
             if (a.out3) {
                 a.out3[3 * static_cast<size_t>(vid)] = rv.x;
                 a.out3[3 * static_cast<size_t>(vid) + 1] = rv.y;
@@ -169,11 +170,12 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int grp, int la
     float du[3] = {0.f, 0.f, 0.f};
     if (lane == 0) {
         const float4 pv = gload(static_cast<const float4*>(P), vid);
-        const dvec3 p = dunpack(pv);
+        const float un = a.ps.lo_unit;
+        const dvec3 p = dunpack(pv, un);
         double dx[3];
-        vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(a.ps.last_p), vid)), dunpack(gload(a.y, vid)),
-                        gload(a.ps.m, vid), a.damping, a.dt, dx);
-        const float4 out = dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]});
+        vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(a.ps.last_p), vid), un),
+                        dunpack(gload(a.y, vid), un), gload(a.ps.m, vid), a.damping, a.dt, dx);
+        const float4 out = dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}, a.ps.lo_inv);
         gstore(a.mode == GATHER_VBD_JACOBI ? a.p_new : a.ps.p, vid, out);
         // the move of the full value (the cached gradients are built from
         // hi + lo); zero when the solve left the vertex where it was
@@ -256,8 +258,8 @@ template <int Q>
 constexpr int refresh_epb() { return 256 / Q; }
 
 template <int S, int Q>
-__device__ __forceinline__ void ju_refresh_block(const float4* __restrict__ P, const FemView& f, float* ju, int e0,
-                                                 int ne) {
+__device__ __forceinline__ void ju_refresh_block(const float4* __restrict__ P, float unit, const FemView& f, float* ju,
+                                                 int e0, int ne) {
     constexpr int EPB = refresh_epb<Q>();
     __shared__ float4 sx[EPB * S];
     for (int t = threadIdx.x; t < ne * S; t += blockDim.x) sx[t] = P[f.topo[static_cast<size_t>(e0) * S + t]];
@@ -268,7 +270,7 @@ __device__ __forceinline__ void ju_refresh_block(const float4* __restrict__ P, c
     float xs[S][3];
     const float4 ref = sx[le * S];
 #pragma unroll
-    for (int k = 0; k < S; ++k) rel_node(sx[le * S + k], ref, xs[k]);
+    for (int k = 0; k < S; ++k) rel_node(sx[le * S + k], ref, unit, xs[k]);
     float Ju[9];
     jx_from_nodes<S>(xs, f.dN + q * S * 3, Ju);
     float* g = ju + (static_cast<size_t>(e0 + le) * Q + q) * 9;
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(256) void k_ju_refresh(PSView ps, FemView f, float*
     constexpr int EPB = refresh_epb<Q>();
     const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
     const int ne = min(EPB, f.E - e0);
-    ju_refresh_block<S, Q>(ps.p, f, ju, e0, ne);
+    ju_refresh_block<S, Q>(ps.p, ps.lo_unit, f, ju, e0, ne);
 }
 
 // Batched form: VSeg.count is the scene's element count, each block takes
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void k_batch_ju_refresh(BatchRefreshArgs b) {
     if (b.iteration >= sd.iterations) return;      // uniform over the block
     const int e0 = (blk - seg.block_begin) * EPB;
     const int ne = min(EPB, seg.count - e0);
-    ju_refresh_block<S, Q>(sd.ps.p, sd.fem, sd.ju, e0, ne);
+    ju_refresh_block<S, Q>(sd.ps.p, sd.ps.lo_unit, sd.fem, sd.ju, e0, ne);
 }
 
 #if !defined(OE_GATHER_S) || OE_GATHER_S != 27   // 27-node hexahedra have their own task kernels
@@ -394,7 +396,7 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
             h[k] = gload(static_cast<const float4*>(ps.p), gload(f.topo, static_cast<size_t>(pk >> 5) * S + k));
         float xs[S][3];
 #pragma unroll
-        for (int k = 0; k < S; ++k) rel_node(h[k], h[0], xs[k]);
+        for (int k = 0; k < S; ++k) rel_node(h[k], h[0], ps.lo_unit, xs[k]);
         const int eid = pk >> 5, r = pk & 31;
         Acc9 acc;
         elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);

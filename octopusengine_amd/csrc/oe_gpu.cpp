@@ -10,6 +10,7 @@
 #include "oe_gpu.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -64,6 +65,15 @@ GPU_ParticleSystem::GPU_ParticleSystem(Engine* e, const std::vector<float>& xyz,
     for (int i = 0; i < n; ++i) inv[i] = 1.f / masses[i];   // w = 1/m (GPU_ParticleSystem.cu:11-12)
     MemoryPool* pool = &eng->pool;
     rest = xyz;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double x = xyz[3 * static_cast<size_t>(i) + k];
+            lo[k] = i ? std::min(lo[k], x) : x;
+            hi[k] = i ? std::max(hi[k], x) : x;
+        }
+    lo_unit = lo_unit_for(std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                    (hi[2] - lo[2]) * (hi[2] - lo[2])));
     p = DBuf<float4>(pool, zero);        // displacement from the rest position
     last_p = DBuf<float4>(pool, zero);
     init_p = DBuf<float4>(pool, X);
@@ -75,8 +85,8 @@ GPU_ParticleSystem::GPU_ParticleSystem(Engine* e, const std::vector<float>& xyz,
 }
 
 PSView GPU_ParticleSystem::get_parameters() const {
-    return PSView{n,       p.get(),       last_p.get(), init_p.get(), v.get(), f.get(), m.get(), w.get(), mask.get(),
-                  comp ? 1 : 0};
+    return PSView{n,          p.get(),    last_p.get(), init_p.get(), v.get(), f.get(), m.get(), w.get(), mask.get(),
+                  comp ? 1 : 0, lo_unit,    1.f / lo_unit};
 }
 
 // the records' w lanes are zero until a compensated integrator writes them
@@ -87,7 +97,7 @@ std::vector<double> GPU_ParticleSystem::full(const DBuf<float4>& rec) const {
     rec.download(h, eng->stream);
     std::vector<double> u(3 * h.size());
     for (size_t i = 0; i < h.size(); ++i) {
-        const dvec3 d = comp ? dunpack(h[i]) : dvec3{h[i].x, h[i].y, h[i].z};
+        const dvec3 d = comp ? dunpack(h[i], lo_unit) : dvec3{h[i].x, h[i].y, h[i].z};
         u[3 * i] = d.x;
         u[3 * i + 1] = d.y;
         u[3 * i + 2] = d.z;
@@ -114,7 +124,7 @@ void GPU_ParticleSystem::upload_split(const float* xyz, DBuf<float4>& rec) {
     for (int i = 0; i < n; ++i) {
         const dvec3 u{static_cast<double>(xyz[3 * i]) - rest[3 * i], static_cast<double>(xyz[3 * i + 1]) - rest[3 * i + 1],
                       static_cast<double>(xyz[3 * i + 2]) - rest[3 * i + 2]};
-        h[i] = comp ? dpack(u)
+        h[i] = comp ? dpack(u, 1.f / lo_unit)
                     : make_float4(static_cast<float>(u.x), static_cast<float>(u.y), static_cast<float>(u.z), 0.f);
     }
     rec.upload(h, eng->stream);

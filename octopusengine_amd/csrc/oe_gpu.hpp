@@ -74,6 +74,7 @@ struct GPU_ParticleSystem {
 
 private:
     bool comp = false;
+    float lo_unit = 1.f;   // quantum of the low-order parts (lo_unit_for the rest bounding box)
     // displacement of every vertex, hi + lo in double (lo = 0 without compensation)
     std::vector<double> full(const DBuf<float4>& rec) const;
     void upload_split(const float* xyz, DBuf<float4>& rec);

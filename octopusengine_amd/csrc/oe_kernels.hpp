@@ -245,9 +245,9 @@ __device__ __forceinline__ void solve_slot_from_lds(const PSView& ps, const Slot
         for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
     }
     double dx[3];
-    const dvec3 p = dunpack(st.p);
-    vbd_local_solve(acc, p, dunpack(st.last_p), dunpack(st.y), st.m, damping, dt, dx);
-    gstore(ps.p, st.vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}));
+    const dvec3 p = dunpack(st.p, ps.lo_unit);
+    vbd_local_solve(acc, p, dunpack(st.last_p, ps.lo_unit), dunpack(st.y, ps.lo_unit), st.m, damping, dt, dx);
+    gstore(ps.p, st.vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}, ps.lo_inv));
 }
 
 // Phase 2 of the fused task kernels for one slot: add the slot's partials
@@ -264,11 +264,12 @@ __device__ __forceinline__ void solve_slot_direct(const PSView& ps, const Owners
 #pragma unroll
         for (int i = 0; i < 9; ++i) acc.v[i] += part[j][i];
     }
-    const dvec3 p = dunpack(gload(static_cast<const float4*>(ps.p), vid));
+    const float un = ps.lo_unit;
+    const dvec3 p = dunpack(gload(static_cast<const float4*>(ps.p), vid), un);
     double dx[3];
-    vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(ps.last_p), vid)), dunpack(gload(y, vid)),
+    vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(ps.last_p), vid), un), dunpack(gload(y, vid), un),
                     gload(ps.m, vid), damping, dt, dx);
-    gstore(ps.p, vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}));
+    gstore(ps.p, vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}, ps.lo_inv));
 }
 
 }  // namespace oe

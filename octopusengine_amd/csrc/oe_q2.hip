@@ -139,7 +139,7 @@ __device__ __forceinline__ LanePlan lane_plan(const q2::Basis1D& B) {
 // One task: incidence (eid, r), the lane's node displacement u already loaded.
 // Returns the lane's contribution to node r (zero on the idle lanes).
 __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialView& mt, const LanePlan& P, int eid,
-                                             int r, float4 u, TaskLDS& sh, Acc9& acc) {
+                                             int r, float4 u, float unit, TaskLDS& sh, Acc9& acc) {
     // the point's element record and dN_r, loaded while the stages run
     float JXi[9], V;
     load_element_record(f, eid, P.q, JXi, V);
@@ -160,7 +160,7 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
                   : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.z), 0)),
             0.f);
         float d[3];
-        rel_node(u, ref, d);
+        rel_node(u, ref, unit, d);
         sh.u[P.place] = make_float4(d[0], d[1], d[2], 0.f);
     }
     half_sync();
@@ -233,7 +233,7 @@ __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, 
         pk_next = gload(own, min(first + task + 2 * kHalves, last));
         Acc9 acc;
         acc_zero(acc);
-        q2_task_body(f, mt, P, eid, r, u, sh, acc);
+        q2_task_body(f, mt, P, eid, r, u, ps.lo_unit, sh, acc);
         // the half's totals: value 2k in row 0 and value 2k + 1 in row 1 of s[k]
         float s[5];
 #pragma unroll
