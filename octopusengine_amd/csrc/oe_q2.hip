@@ -113,6 +113,15 @@ __device__ __forceinline__ float half_sum_pair(float a, float b) {
     return row_sum(__uint_as_float(r[0]) + __uint_as_float(r[1]));
 }
 
+// Lane 0 of each half, in every lane of that half: v_mov_b32_dpp
+// row_newbcast:0 gives each 16-lane row its first lane's value, then
+// v_permlane16_swap of that with itself puts rows (0, 0, 2, 2) in its first
+// result (lanes 0 and 32).  Two VALU instructions, no branch, no LDS.
+__device__ __forceinline__ float half_bcast0(float x) {
+    const int r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150, 0xF, 0xF, false);
+    return __uint_as_float(__builtin_amdgcn_permlane16_swap(r, r, false, false)[0]);
+}
+
 // Lane-constant part of a half's work: its point's tensor indices, the 1D
 // factors of the three stages and its LDS rows.
 struct LanePlan {
@@ -147,18 +156,9 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
     const float dnr[3] = {gload(f.dN, (P.q * 27 + r) * 3), gload(f.dN, (P.q * 27 + r) * 3 + 1),
                           gload(f.dN, (P.q * 27 + r) * 3 + 2)};
     // stage 0: the node's value relative to node 0 of the element (lane 0 of
-    // the half, read with v_readlane: a half that has left the task loop is
-    // read but not used)
+    // the half; half_bcast0)
     {
-        const bool upper = (threadIdx.x & 32) != 0;
-        const float4 ref = make_float4(
-            upper ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.x), 32))
-                  : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.x), 0)),
-            upper ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.y), 32))
-                  : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.y), 0)),
-            upper ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.z), 32))
-                  : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u.z), 0)),
-            0.f);
+        const float4 ref = make_float4(half_bcast0(u.x), half_bcast0(u.y), half_bcast0(u.z), 0.f);
         float d[3];
         rel_node(u, ref, unit, d);
         sh.u[P.place] = make_float4(d[0], d[1], d[2], 0.f);
