@@ -191,6 +191,34 @@ __global__ __launch_bounds__(kBlock) void k_box(PSView ps, float3 pmin, float3 p
     if (box_clamp(u, ps.init_p[i], lo, hi)) ps.p[i] = u;   // written only when clamped
 }
 
+// Fixation then box limit over one scene's vertices, fused as in the batched
+// form: a fixed vertex (flag set) is snapped and masked, every other vertex
+// with mask 1 is clamped; the same result as k_fix then k_box, since the box
+// skips masked vertices.
+__global__ __launch_bounds__(kBlock) void k_fix_box(PSView ps, const unsigned char* fixed, FixTransform t, float3 pmin,
+                                                    float3 pmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ps.n) return;
+    const bool is_fixed = gload(fixed, i) != 0;
+    const int mask = gload(static_cast<const int*>(ps.mask), i);
+    float4 u = gload(static_cast<const float4*>(ps.p), i);
+    const float4 X = gload(static_cast<const float4*>(ps.init_p), i);
+    if (is_fixed) {
+        gstore(ps.p, i, fix_displacement(X, t.com, t.offset, t.R));
+        gstore(ps.v, i, f4(0.f, 0.f, 0.f, 0.f));
+        gstore(ps.f, i, f4(0.f, 0.f, 0.f, 0.f));
+        gstore(ps.mask, i, 0);
+        return;
+    }
+    if (mask == 0) return;
+    const float lo[3] = {pmin.x, pmin.y, pmin.z}, hi[3] = {pmax.x, pmax.y, pmax.z};
+    if (ps.comp) {
+        box_clamp_comp(ps, i, X, lo, hi);
+        return;
+    }
+    if (box_clamp(u, X, lo, hi)) gstore(ps.p, i, u);   // written only when clamped
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* slot_vid, int first, int count,
                                                         const float4* p_new) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -301,6 +329,13 @@ void launch_pbd_velocity(const PSView& ps, float dt, float global_damping, hipSt
 void launch_fix(const PSView& ps, const int* ids, int n, const FixTransform& t, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fix, dim3(grid_for(n)), dim3(kBlock), 0, s, ps, ids, n, t);
+}
+
+void launch_fix_box(const PSView& ps, const unsigned char* fixed, const FixTransform& t, const float pmin[3],
+                    const float pmax[3], hipStream_t s) {
+    if (ps.n <= 0) return;
+    hipLaunchKernelGGL(k_fix_box, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, fixed, t,
+                       make_float3(pmin[0], pmin[1], pmin[2]), make_float3(pmax[0], pmax[1], pmax[2]));
 }
 
 void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipStream_t s) {

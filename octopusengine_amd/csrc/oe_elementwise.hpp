@@ -21,6 +21,9 @@ void launch_semi_implicit(const PSView& ps, float dt, const float g[3], hipStrea
 void launch_pbd_velocity(const PSView& ps, float dt, float global_damping, hipStream_t s);
 void launch_fix(const PSView& ps, const int* ids, int n, const FixTransform& t, hipStream_t s);
 void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipStream_t s);
+// launch_fix then launch_box in one pass over all vertices; fixed = per-vertex flag
+void launch_fix_box(const PSView& ps, const unsigned char* fixed, const FixTransform& t, const float pmin[3],
+                    const float pmax[3], hipStream_t s);
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,
                         hipStream_t s);
 

@@ -172,9 +172,23 @@ void GPU_Integrator::eval_dynamics(GPU_ParticleSystem* ps, float dt) {
     for (auto& d : _dynamics)
         if (d->active) d->step(ps, dt);
 }
+// A fixation directly followed by a box limit (the order a scene adds them in)
+// runs as one fused pass; everything else in insertion order.
 void GPU_Integrator::eval_constraints(GPU_ParticleSystem* ps, float dt) {
-    for (auto& c : _constraints)
-        if (c->active) c->step(ps, dt);
+    for (size_t i = 0; i < _constraints.size(); ++i) {
+        GPU_Dynamic* c = _constraints[i].get();
+        if (!c->active) continue;
+        if (i + 1 < _constraints.size() && _constraints[i + 1]->active) {
+            auto* fix = dynamic_cast<GPU_Fix_Constraint*>(c);
+            auto* box = dynamic_cast<GPU_Box_Limit*>(_constraints[i + 1].get());
+            if (fix && box) {
+                launch_fix_box(ps->get_parameters(), fix->d_flags.get(), fix->t, box->pmin, box->pmax, fix->stream);
+                ++i;
+                continue;
+            }
+        }
+        c->step(ps, dt);
+    }
 }
 
 void GPU_SemiExplicit::step(GPU_ParticleSystem* ps, float dt) {
@@ -597,6 +611,9 @@ GPU_Fix_Constraint::GPU_Fix_Constraint(Engine* e, const std::vector<float>& xyz,
     const float I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     std::memcpy(t.R, I, sizeof(I));
     d_ids = DBuf<int>(&e->pool, ids);
+    std::vector<unsigned char> flags(xyz.size() / 3, 0);
+    for (int i : ids) flags[i] = 1;
+    d_flags = DBuf<unsigned char>(&e->pool, flags);
 }
 
 void GPU_Fix_Constraint::step(GPU_ParticleSystem* ps, float) {

@@ -272,6 +272,7 @@ struct GPU_Fix_Constraint final : GPU_Dynamic {
     FixTransform t;
     std::vector<int> ids;
     DBuf<int> d_ids;
+    DBuf<unsigned char> d_flags;   // per vertex: 1 when fixed (the fused fix + box pass)
     hipStream_t stream;
 };
 
