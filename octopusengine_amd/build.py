@@ -33,7 +33,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]
 
 HOST_SOURCES = ["oe_shapes.cpp", "oe_mesh.cpp", "oe_color.cpp", "oe_fem.cpp", "oe_vtk.cpp", "oe_loader.cpp",
                 "oe_record.cpp"]
-HIP_SOURCES = ["oe_gpu.cpp", "oe_scene.cpp", "oe_batch.cpp", "oe_server.cpp", "oe_capi.cpp", "oe_elementwise.hip", "oe_xpbd.hip", "oe_diag.hip", "oe_q2.hip", "oe_scratch.hip", "oe_tune.cpp"]
+HIP_SOURCES = ["oe_gpu.cpp", "oe_scene.cpp", "oe_batch.cpp", "oe_server.cpp", "oe_capi.cpp", "oe_elementwise.hip", "oe_xpbd.hip", "oe_diag.hip", "oe_q2.hip", "oe_q2p.hip", "oe_scratch.hip", "oe_tune.cpp"]
 GATHER_VARIANTS = [(4, 1), (8, 8), (10, 4), (20, 11), (27, 27)]
 EXTRA_HIP_SOURCES: list[str] = []
 

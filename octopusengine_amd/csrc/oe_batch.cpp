@@ -158,6 +158,7 @@ void Schedule::build(const std::vector<Scene*>& scenes, const std::vector<int>& 
                     gl.L = t.L;
                     gl.QL = t.QL;
                     gl.mode = t.mode;
+                    gl.inc = t.inc;
                     any = true;
                     if (gl.mode == PASS_TASKS) q2_basis = s->vbd_fem->q2_basis;
                 }
@@ -290,7 +291,10 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                     tk.seg_first = gl.seg_first;
                     tk.seg_count = gl.seg_count;
                     tk.latency = gl.latency ? 1 : 0;
-                    launch_batch_q2_fused(tk, gl.nblocks, stream);
+                    if (!gl.latency && q2_pencil(gl.inc))
+                        launch_batch_q2_pencil(tk, gl.nblocks, stream);
+                    else
+                        launch_batch_q2_fused(tk, gl.nblocks, stream);
                 } else if (gl.mode == PASS_TASKS) {
                     ft.seg_first = gl.seg_first;
                     ft.seg_count = gl.seg_count;

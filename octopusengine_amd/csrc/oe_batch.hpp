@@ -36,6 +36,7 @@ struct Schedule {
         int color, element, S, Q, L, QL;
         int mode;                         // PassMode
         bool latency;                     // task pass in the kernels' latency form
+        int inc = 0;                      // task pass: the tile's incidences per throughput-form block
         int seg_first, seg_count, nblocks;   // gather: slot ranges; tasks: vertex-block ranges
         int64_t slots;   // vertex slots covered (all scenes)
         int64_t incidences;

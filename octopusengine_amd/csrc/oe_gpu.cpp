@@ -294,7 +294,7 @@ GatherTile default_tile(Element e) {
         case Hexa: return {16, 1, PASS_GATHER};
         case Tetra10: return {64, 1, PASS_TASKS};
         case Tetra20: return {64, 1, PASS_GATHER};
-        case Hexa27: return {64, 1, PASS_TASKS};
+        case Hexa27: return {64, 1, PASS_TASKS, kQ2PencilIncPerBlock};
         default: return {16, 1, PASS_GATHER};
     }
 }
@@ -508,10 +508,13 @@ void GPU_VBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
             const TaskTables& tt = lat ? *lt : *thr;
             const int4* blocks = tt.blocks.get();
             const int b0 = tt.color_begin[c], nb = tt.color_blocks(c);
-            if (element == Hexa27)
-                launch_q2_fused(Q2FusedArgs{a.ps, a.fem, a.own, a.mt, q2_basis, y, blocks, damping, dt, b0, nb,
-                                            lat ? 1 : 0},
-                                s);
+            if (element == Hexa27) {
+                const Q2FusedArgs qa{a.ps, a.fem, a.own, a.mt, q2_basis, y, blocks, damping, dt, b0, nb, lat ? 1 : 0};
+                if (!lat && q2_pencil(tile.inc))
+                    launch_q2_pencil(qa, s);
+                else
+                    launch_q2_fused(qa, s);
+            }
             else if (!launch_fused_tasks(S, Q, FusedTaskArgs{a.ps, a.fem, a.own, a.mt, y, blocks, damping, dt, b0, nb,
                                                              lat ? 1 : 0},
                                          s))

@@ -23,6 +23,12 @@ constexpr int kQ2FusedThreads = 256;
 constexpr int kQ2FusedIncPerBlock = 32;
 constexpr int kQ2FusedIncPerBlockLat = 8;
 constexpr int kQ2FusedMaxSlots = 64;
+// The pencil form of the throughput pass (oe_q2p.hip, the default): one-wave
+// blocks of 7 tasks at a time, in 4 rounds; chosen by a tile of this many
+// incidences per block.  Other tile sizes take the 27-lane throughput form.
+constexpr int kQ2PencilIncPerBlock = 28;
+constexpr int kQ2PencilMaxSlots = kQ2PencilIncPerBlock;
+inline bool q2_pencil(int inc) { return inc == kQ2PencilIncPerBlock; }
 
 struct Q2FusedArgs {
     PSView ps;
@@ -54,5 +60,8 @@ struct BatchQ2FusedArgs {
 
 void launch_q2_fused(const Q2FusedArgs& a, hipStream_t s);
 void launch_batch_q2_fused(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s);
+// throughput form with pencil tasks (blocks of at most kQ2PencilIncPerBlock incidences)
+void launch_q2_pencil(const Q2FusedArgs& a, hipStream_t s);
+void launch_batch_q2_pencil(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s);
 
 }  // namespace oe

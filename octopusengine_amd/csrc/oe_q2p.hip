@@ -1,0 +1,264 @@
+// Throughput form of the Q2 VBD colour pass with "pencil" tasks: a task
+// (incidence: element e, local node r) takes 9 lanes, lane m = (j, k) owning
+// the three quadrature points (i, j, k), i = 0..2, and a wave runs 7 tasks at
+// once (lanes 0-62; lane 63 idle).  A block is one wave: every exchange is
+// inside the wave, so the block needs no workgroup barrier.
+//
+// Per task, with u_abc the node values at tensor place (a, b, c) and l, l'
+// the 1D Lagrange factors and their derivatives at the Gauss coordinates:
+//   stage 0  lane (b, c) loads nodes (a, b, c), a = 0..2, into LDS, as
+//            (hi - hi of node 0) + lo (rel_node)
+//   stage A  lane (b, k):  T_ab(k) = sum_c l_c(z_k) u_abc,  Tz_ab(k) = sum_c l_c'(z_k) u_abc
+//   stage B  lane (j, k):  X_a = sum_b l_b(y_j) T_ab,  Xy_a = sum_b l_b'(y_j) T_ab,
+//                          Xz_a = sum_b l_b(y_j) Tz_ab                    (in registers)
+//   stage C  lane (j, k), i = 0..2:  dU/dx = sum_a l_a'(x_i) X_a,  dU/dy = sum_a l_a(x_i) Xy_a,
+//                          dU/dz = sum_a l_a(x_i) Xz_a, then the point's force / Hessian
+//                          contribution to node r (nh_contrib), summed over the lane's 3 points
+//   sum      the group's 9 lanes through LDS, into the block's partials
+// - 72 FMAs per point for the gradient (108 in the 27-lane form, whose stage 1
+// recomputes each c-contraction on three lanes), and two LDS exchanges per
+// task instead of two per point.  After the tasks, one thread per slot adds
+// its vertex's partials in the owners order and runs the FP64 local solve.
+#include "oe_kernels.hpp"
+#include "oe_q2.hpp"
+
+namespace oe {
+
+namespace {
+
+constexpr int kGroups = 7;   // tasks per wave
+static_assert(kQ2PencilIncPerBlock % kGroups == 0, "whole rounds of tasks");
+constexpr int kRounds = kQ2PencilIncPerBlock / kGroups;
+static_assert(kQ2PencilMaxSlots <= 64, "phase 2 runs on the block's one wave");
+
+// node index of tensor place t = a * 9 + b * 3 + c (quadrature points use the
+// same numbering), and the tensor place of node n
+struct PencilTables {
+    int node[27];
+    int place[27];
+};
+constexpr PencilTables make_pencil_tables() {
+    PencilTables t{};
+    for (int p = 0; p < 27; ++p) {
+        t.node[p] = q2::tensor_to_node(p);
+        t.place[p] = q2::kNodeABC[p];
+    }
+    return t;
+}
+__constant__ PencilTables kPencil = make_pencil_tables();
+
+// A task group's LDS (243 floats): the node values, then (once stage A has
+// read them) the lanes' sums; T and Tz.
+struct GroupLDS {
+    float r1[81];   // stage 0: d[(a * 9 + b * 3 + c) * 3 + comp]; sum: s[m * 9 + v]
+    float t[81];    // T_ab(k) at [(a * 9 + b * 3 + k) * 3 + comp]
+    float tz[81];   // Tz_ab(k)
+};
+
+// Waiting for the wave's own LDS operations orders them for every lane of the
+// wave (a wave's LDS operations complete in order); the compiler must not move
+// memory accesses across.
+__device__ __forceinline__ void wave_sync() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ float l_at(const q2::Basis1D& B, int a, int i) { return B.L[a][i]; }
+__device__ __forceinline__ float d_at(const q2::Basis1D& B, int a, int i) { return B.D[a][i]; }
+__device__ __forceinline__ float sel3(int k, float v0, float v1, float v2) { return k == 0 ? v0 : (k == 1 ? v1 : v2); }
+
+// One round of the block's tasks: task = round * 7 + g for the lane's group g.
+// wq_i: the weights of the lane's points (i, j, k), i = 0..2.
+__device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f, const int* __restrict__ own,
+                                             const MaterialView& mt, const q2::Basis1D& B, const float (&wq_i)[3],
+                                             int first_own, int n_inc, int round, int g, int m, GroupLDS& sh,
+                                             float (*part)[9]) {
+    const int task = round * kGroups + g;
+    const bool act = g < kGroups && task < n_inc;
+    const int p1 = m / 3, p2 = m - 3 * p1;   // stage 0 / A: (b, c) and (b, k); stage B / C: (j, k)
+    int eid = 0, r = 0;
+    float4 u[3];
+    if (act) {
+        const int pk = gload(own, first_own + task);
+        eid = pk >> 5;
+        r = pk & 31;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            u[a] = gload(static_cast<const float4*>(ps.p),
+                         gload(f.topo, static_cast<size_t>(eid) * 27 + kPencil.node[a * 9 + m]));
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) u[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // the element's record: JX_inv and |det J| (per point when not affine:
+    // point (0, j, k) here, the others below)
+    float JXi[9], V0 = 0.f, detB = 0.f;
+    if (act) {
+        if (f.affine) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) JXi[i] = gload(f.jxv, static_cast<size_t>(eid) * 10 + i);
+            detB = gload(f.jxv, static_cast<size_t>(eid) * 10 + 9);
+        } else {
+            load_element_record(f, eid, kPencil.node[m], JXi, V0);
+        }
+    }
+    // stage 0: node 0 (tensor place 0) is the a = 0 node of lane m = 0 of the group
+    {
+        const int src = (g < kGroups ? g : 0) * 9;
+        const float4 ref = make_float4(__shfl(u[0].x, src), __shfl(u[0].y, src), __shfl(u[0].z, src), 0.f);
+        if (act) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float d[3];
+                rel_node(u[a], ref, ps.lo_unit, d);
+                float* o = &sh.r1[(a * 9 + m) * 3];
+                o[0] = d[0];
+                o[1] = d[1];
+                o[2] = d[2];
+            }
+        }
+    }
+    wave_sync();
+    // stage A: lane (b, k) = (p1, p2)
+    if (act) {
+        const float lc[3] = {l_at(B, 0, p2), l_at(B, 1, p2), l_at(B, 2, p2)};
+        const float dc[3] = {d_at(B, 0, p2), d_at(B, 1, p2), d_at(B, 2, p2)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float* x0 = &sh.r1[(a * 9 + p1 * 3 + 0) * 3];
+            const float* x1 = &sh.r1[(a * 9 + p1 * 3 + 1) * 3];
+            const float* x2 = &sh.r1[(a * 9 + p1 * 3 + 2) * 3];
+            float* t = &sh.t[(a * 9 + m) * 3];
+            float* tz = &sh.tz[(a * 9 + m) * 3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                t[c] = lc[0] * x0[c] + lc[1] * x1[c] + lc[2] * x2[c];
+                tz[c] = dc[0] * x0[c] + dc[1] * x1[c] + dc[2] * x2[c];
+            }
+        }
+    }
+    wave_sync();
+    // stage B: lane (j, k) = (p1, p2); X, Xy, Xz for a = 0..2 in registers
+    float X[3][3], Xy[3][3], Xz[3][3];
+    {
+        const float lb[3] = {l_at(B, 0, p1), l_at(B, 1, p1), l_at(B, 2, p1)};
+        const float db[3] = {d_at(B, 0, p1), d_at(B, 1, p1), d_at(B, 2, p1)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float* t0 = &sh.t[(a * 9 + 0 * 3 + p2) * 3];
+            const float* t1 = &sh.t[(a * 9 + 1 * 3 + p2) * 3];
+            const float* t2 = &sh.t[(a * 9 + 2 * 3 + p2) * 3];
+            const float* z0 = &sh.tz[(a * 9 + 0 * 3 + p2) * 3];
+            const float* z1 = &sh.tz[(a * 9 + 1 * 3 + p2) * 3];
+            const float* z2 = &sh.tz[(a * 9 + 2 * 3 + p2) * 3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[a][c] = lb[0] * t0[c] + lb[1] * t1[c] + lb[2] * t2[c];
+                Xy[a][c] = db[0] * t0[c] + db[1] * t1[c] + db[2] * t2[c];
+                Xz[a][c] = lb[0] * z0[c] + lb[1] * z1[c] + lb[2] * z2[c];
+            }
+        }
+    }
+    // stage C and the material terms at the lane's points (i, j, k).  dN_r is
+    // a product of the 1D factors (the shape table holds the same products:
+    // q2_basis_1d checks it): r at tensor place (ra, rb, rc) has
+    // dN_r(i, j, k) = (l'_ra(x_i) l_rb(y_j) l_rc(z_k), l_ra l'_rb l_rc, l_ra l_rb l'_rc).
+    Acc9 acc;
+    acc_zero(acc);
+    if (act) {
+        const int tr = kPencil.place[r];
+        const int ra = tr / 9, rb = tr / 3 % 3, rc = tr % 3;
+        const float lrb = sel3(rb, l_at(B, 0, p1), l_at(B, 1, p1), l_at(B, 2, p1));
+        const float drb = sel3(rb, d_at(B, 0, p1), d_at(B, 1, p1), d_at(B, 2, p1));
+        const float lrc = sel3(rc, l_at(B, 0, p2), l_at(B, 1, p2), l_at(B, 2, p2));
+        const float drc = sel3(rc, d_at(B, 0, p2), d_at(B, 1, p2), d_at(B, 2, p2));
+        const float ll = lrb * lrc, dl = drb * lrc, ld = lrb * drc;
+        // one point at a time (unrolled, the three points' material terms
+        // interleave and hold registers that cost a wave per SIMD)
+#pragma unroll 1
+        for (int i = 0; i < 3; ++i) {
+            const int q = kPencil.node[i * 9 + m];
+            float Ju[9];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Ju[c] = d_at(B, 0, i) * X[0][c] + d_at(B, 1, i) * X[1][c] + d_at(B, 2, i) * X[2][c];
+                Ju[3 + c] = l_at(B, 0, i) * Xy[0][c] + l_at(B, 1, i) * Xy[1][c] + l_at(B, 2, i) * Xy[2][c];
+                Ju[6 + c] = l_at(B, 0, i) * Xz[0][c] + l_at(B, 1, i) * Xz[1][c] + l_at(B, 2, i) * Xz[2][c];
+            }
+            float V;
+            if (f.affine) {
+                V = detB * sel3(i, wq_i[0], wq_i[1], wq_i[2]);
+            } else if (i == 0) {
+                V = V0;
+            } else {
+                load_element_record(f, eid, q, JXi, V);
+            }
+            const float lra = sel3(ra, l_at(B, 0, i), l_at(B, 1, i), l_at(B, 2, i));
+            const float dra = sel3(ra, d_at(B, 0, i), d_at(B, 1, i), d_at(B, 2, i));
+            const float dnr[3] = {dra * ll, lra * dl, lra * ld};
+            nh_contrib(Ju, JXi, V, dnr, mt, acc);
+        }
+    }
+    // sum over the group's 9 lanes: stage A has read r1, so it holds the sums now
+    if (act) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) sh.r1[m * 9 + v] = acc.v[v];
+    }
+    wave_sync();
+    if (act) {
+        float s = 0.f;
+#pragma unroll
+        for (int l = 0; l < 9; ++l) s += sh.r1[l * 9 + m];
+        part[task][m] = s;
+    }
+    wave_sync();   // the next round's stage 0 rewrites r1
+}
+
+__device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fem, const OwnersView& own,
+                                             const MaterialView& mt, const q2::Basis1D& basis, const float4* y,
+                                             float damping, float dt, const int4 blk) {
+    __shared__ GroupLDS sh[kGroups + 1];   // the last one serves lane 63, which stores nothing
+    __shared__ float part[kQ2PencilIncPerBlock][9];
+    const int lane = static_cast<int>(threadIdx.x);
+    const int g = lane / 9, m = lane - 9 * g;
+    // the weights of the lane's points, the same for every task
+    const float wq_i[3] = {gload(fem.wq, kPencil.node[m]), gload(fem.wq, kPencil.node[9 + m]),
+                           gload(fem.wq, kPencil.node[18 + m])};
+    for (int round = 0; round < kRounds && round * kGroups < blk.w; ++round)
+        pencil_round(ps, fem, own.own, mt, basis, wq_i, blk.z, blk.w, round, g, m, sh[g], part);
+    if (lane < blk.y) solve_slot_direct(ps, own, y, damping, dt, blk.x + lane, part, blk.z);
+}
+
+__global__ __launch_bounds__(64) void k_q2_pencil(Q2FusedArgs a) {
+    const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
+    pencil_block(a.ps, a.fem, a.own, a.mt, a.basis, a.y, a.damping, a.dt, blk);
+}
+
+__global__ __launch_bounds__(64) void k_batch_q2_pencil(BatchQ2FusedArgs b) {
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, bid);
+    const SceneDev& sd = b.scenes[seg.scene];
+    if (b.iteration >= sd.iterations) return;   // uniform over the block
+    const PSView ps = sd.ps;
+    const FemView fem = sd.fem;
+    const OwnersView own = sd.own;
+    const MaterialView mt = sd.mt;
+    const float4* y = sd.y;
+    const float damping = sd.damping;
+    const int4 blk = gload(sd.fblocks, seg.first + (bid - seg.block_begin));
+    pencil_block(ps, fem, own, mt, b.basis, y, damping, b.dt, blk);
+}
+
+}  // namespace
+
+void launch_q2_pencil(const Q2FusedArgs& a, hipStream_t s) {
+    if (a.count > 0) hipLaunchKernelGGL(k_q2_pencil, dim3(a.count), dim3(64), 0, s, a);
+}
+
+void launch_batch_q2_pencil(const BatchQ2FusedArgs& b, int nblocks, hipStream_t s) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_batch_q2_pencil, dim3(nblocks), dim3(64), 0, s, b);
+}
+
+}  // namespace oe

@@ -154,7 +154,9 @@ std::vector<GatherTile> tile_candidates(Element el, bool vbd, bool tasks_ok) {
     for (int L : {4, 8, 16, 32, 64}) out.push_back(GatherTile{L, 1, PASS_CACHE});
     if (!tasks_ok) return out;
     const int maxi = max_task_inc(el);
-    for (int inc : {maxi, maxi / 2, maxi / 4})
+    std::vector<int> incs{maxi, maxi / 2, maxi / 4};
+    if (el == Hexa27) incs.push_back(kQ2PencilIncPerBlock);   // the pencil form of the throughput pass
+    for (int inc : incs)
         for (int64_t lat : {int64_t{0}, int64_t{256}, kLatencyBlocks, int64_t{2048},
                             std::numeric_limits<int64_t>::max()}) {
             GatherTile t{64, 1, PASS_TASKS};
