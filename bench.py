@@ -48,6 +48,8 @@ if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# incidences per block of the Q2 tiles that select the pencil form of the throughput pass
+Q2_PENCIL_INC = (14, 21, 28)
 SIZE = (1.0, 0.25, 0.25)
 DT = 0.005
 ITERS = 10
@@ -180,7 +182,9 @@ def batched_leg(args, eng) -> dict:
     # launch, on the stream the kernels run on (kernel durations only)
     b.set_timing(True)
     b.get_timing()
+    q2_pencil = b.get_tile_ex(oe.HEXA27)["inc_per_block"] in Q2_PENCIL_INC
     kinds = [(oe.TETRA10, "k_batch_fused_tasks<10,4> (P2 colour pass)", "k_batch_fused_tasks<10, 4, false>"),
+             (oe.HEXA27, "k_batch_q2_pencil (Q2 colour pass, pencil tasks)", "k_batch_q2_pencil") if q2_pencil else
              (oe.HEXA27, "k_batch_q2_fused (Q2 colour pass)", "void k_batch_q2_fused<false>")]
     for el, _, _ in kinds:
         b.get_timing_element(el)

@@ -23,9 +23,11 @@ GPU_Box_Limit* find_box(const Scene& s) {
 }
 }  // namespace
 
-std::map<int, GatherTile> default_tiles() {
+std::map<int, GatherTile> default_tiles(bool served) {
     std::map<int, GatherTile> t;
     for (Element e : {Tetra, Hexa, Tetra10, Tetra20, Hexa27}) t[e] = default_tile(e);
+    // a batch's launches hold many blocks: the longer pencil blocks
+    if (!served) t[Hexa27].inc = kQ2PencilIncPerBlock;
     return t;
 }
 

@@ -103,7 +103,9 @@ struct GatherTimer {
 void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const SceneDev* d_scenes, float dt, int substeps,
                      hipStream_t stream, GatherTimer* timer);
 
-std::map<int, GatherTile> default_tiles();
+// The default tile of every element type: a scene's (default_tile) for a
+// server's ticks, and for a batch the same but 28-incidence Q2 pencil blocks.
+std::map<int, GatherTile> default_tiles(bool served = false);
 
 // A batch registers itself with its scenes (Scene::batches).  A scene may be
 // in several batches but appear once per batch, and a scene held by a server

@@ -294,7 +294,7 @@ GatherTile default_tile(Element e) {
         case Hexa: return {16, 1, PASS_GATHER};
         case Tetra10: return {64, 1, PASS_TASKS};
         case Tetra20: return {64, 1, PASS_GATHER};
-        case Hexa27: return {64, 1, PASS_TASKS, kQ2PencilIncPerBlock};
+        case Hexa27: return {64, 1, PASS_TASKS, kQ2PencilIncLatency};
         default: return {16, 1, PASS_GATHER};
     }
 }
@@ -323,6 +323,7 @@ GPU_VBD_FEM::GPU_VBD_FEM(Engine* e, Element el, const HostMesh& mesh, Material m
         throw std::logic_error("DSAT produced a conflicting colouring");
     owners = build_owners(mesh, colors, true);
     upload_owners(eng, owners, d_slot_vid, d_slot_nbo, d_own);
+    for (int32_t nb : owners.slot_nb) max_incidences = std::max(max_incidences, static_cast<int>(nb));
     if (version == Better_Coloration) p_new.allocate(&eng->pool, mesh.nb_vertices());
     if (el == Hexa27) has_q2_basis = q2_basis_1d(q2_basis);
     // allocated here, not in step(), which may run under HIP graph capture
@@ -356,10 +357,15 @@ void upload_vertex_blocks(Engine* eng, const OwnersCSR& o, int max_inc, int max_
 }
 }  // namespace
 
+int GPU_VBD_FEM::fit_inc(int inc) const {
+    const int n = inc > 0 ? inc : (element == Hexa27 ? kQ2FusedIncPerBlock : max_task_inc(element));
+    return element == Hexa27 ? q2_pencil_fit(n, max_incidences) : n;
+}
+
 const TaskTables& GPU_VBD_FEM::throughput_tables(int inc) {
-    const int maxi = max_task_inc(element);
-    const int n = inc > 0 ? inc : maxi;
-    if (n > maxi) throw std::invalid_argument("more incidences per block than the task kernel holds");
+    const int n = fit_inc(inc);
+    if (n > max_task_inc(element) || (element == Hexa27 && !q2_pencil(n) && n > kQ2FusedIncPerBlock))
+        throw std::invalid_argument("more incidences per block than the task kernel holds");
     auto it = thr_tables.find(n);
     if (it != thr_tables.end()) return it->second;
     TaskTables t;
@@ -369,7 +375,7 @@ const TaskTables& GPU_VBD_FEM::throughput_tables(int inc) {
 }
 
 const TaskTables& GPU_VBD_FEM::built_tables(int inc) const {
-    const auto it = thr_tables.find(inc > 0 ? inc : max_task_inc(element));
+    const auto it = thr_tables.find(fit_inc(inc));
     if (it == thr_tables.end()) throw std::logic_error("element-task vertex blocks not prepared");
     return it->second;
 }

@@ -161,7 +161,7 @@ struct GatherTile {
 };
 GatherTile default_tile(Element e);
 // Largest incidences per block the element-task kernels hold.
-inline int max_task_inc(Element e) { return e == Hexa27 ? kQ2FusedIncPerBlock : kFusedIncPerBlock; }
+inline int max_task_inc(Element e) { return e == Hexa27 ? kQ2PencilMaxInc : kFusedIncPerBlock; }
 
 // Vertex blocks of the fused element-task kernels: int4 {first slot, slot
 // count, first owners index, incidence count} per block, colour-major;
@@ -221,6 +221,10 @@ struct GPU_VBD_FEM final : GPU_FEM {
     // incidences per block (0 = the kernel's maximum); throws
     // std::invalid_argument when a vertex has more incidences than a block.
     const TaskTables& throughput_tables(int inc);
+    // the incidences per throughput-form block a tile of inc gets on this mesh
+    // (q2_pencil_fit for 27-node hexahedra; 0 = the kernel's maximum)
+    int fit_inc(int inc) const;
+    int max_incidences = 0;   // most incident elements of one vertex
     // The tables already built for `inc` (std::logic_error if not): for
     // step(), which may run under HIP graph capture and must not allocate.
     const TaskTables& built_tables(int inc) const;

@@ -24,11 +24,27 @@ constexpr int kQ2FusedIncPerBlock = 32;
 constexpr int kQ2FusedIncPerBlockLat = 8;
 constexpr int kQ2FusedMaxSlots = 64;
 // The pencil form of the throughput pass (oe_q2p.hip, the default): one-wave
-// blocks of 7 tasks at a time, in 4 rounds; chosen by a tile of this many
-// incidences per block.  Other tile sizes take the 27-lane throughput form.
+// blocks of 7 tasks at a time, in up to 5 rounds; chosen by a tile of 14, 21,
+// 28 or 35 incidences per block.  Other tile sizes (up to 32) take the 27-lane
+// throughput form.
+//  * 28 (4 rounds): batches, whose launches hold enough blocks to cover the
+//    longer blocks (256-scene step 47.2 ms; 51.4 ms with the 27-lane form).
+//  * 14 (2 rounds): single scenes and the server, whose launches are short of
+//    blocks (served p99 6.6 ms; 7.8 ms at 28 and 7.0 ms with the 27-lane form).
+// A scene whose vertices have more incident elements than the tile's blocks
+// hold takes the next size up (q2_pencil_fit): the kernel runs blocks of any
+// of these sizes, so scenes of one launch may differ.
 constexpr int kQ2PencilIncPerBlock = 28;
-constexpr int kQ2PencilMaxSlots = kQ2PencilIncPerBlock;
-inline bool q2_pencil(int inc) { return inc == kQ2PencilIncPerBlock; }
+constexpr int kQ2PencilIncLatency = 14;
+constexpr int kQ2PencilMaxInc = 35;
+constexpr int kQ2PencilMaxSlots = kQ2PencilMaxInc;
+inline bool q2_pencil(int inc) { return inc >= 14 && inc <= kQ2PencilMaxInc && inc % 7 == 0; }
+// the pencil block size for a tile of inc when a vertex has up to max_incidences
+inline int q2_pencil_fit(int inc, int max_incidences) {
+    if (!q2_pencil(inc)) return inc;
+    while (inc < max_incidences && inc < kQ2PencilMaxInc) inc += 7;
+    return inc;
+}
 
 struct Q2FusedArgs {
     PSView ps;

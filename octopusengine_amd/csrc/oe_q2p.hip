@@ -27,8 +27,8 @@ namespace oe {
 namespace {
 
 constexpr int kGroups = 7;   // tasks per wave
-static_assert(kQ2PencilIncPerBlock % kGroups == 0, "whole rounds of tasks");
-constexpr int kRounds = kQ2PencilIncPerBlock / kGroups;
+static_assert(kQ2PencilMaxInc % kGroups == 0, "whole rounds of tasks");
+constexpr int kRounds = kQ2PencilMaxInc / kGroups;
 static_assert(kQ2PencilMaxSlots <= 64, "phase 2 runs on the block's one wave");
 
 // node index of tensor place t = a * 9 + b * 3 + c (quadrature points use the
@@ -220,7 +220,7 @@ __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fe
                                              const MaterialView& mt, const q2::Basis1D& basis, const float4* y,
                                              float damping, float dt, const int4 blk) {
     __shared__ GroupLDS sh[kGroups + 1];   // the last one serves lane 63, which stores nothing
-    __shared__ float part[kQ2PencilIncPerBlock][9];
+    __shared__ float part[kQ2PencilMaxInc][9];
     const int lane = static_cast<int>(threadIdx.x);
     const int g = lane / 9, m = lane - 9 * g;
     // the weights of the lane's points, the same for every task

@@ -5,7 +5,7 @@
 
 namespace oe {
 
-Server::Server(Engine* eng, int max_batch, int nstreams) : eng_(eng), max_batch_(max_batch), tiles_(default_tiles()) {
+Server::Server(Engine* eng, int max_batch, int nstreams) : eng_(eng), max_batch_(max_batch), tiles_(default_tiles(true)) {
     if (max_batch < 1 || nstreams < 1 || nstreams > 8) throw std::invalid_argument("max_batch >= 1, 1 <= streams <= 8");
     slots_.resize(nstreams);
     for (StreamSlot& s : slots_) {

@@ -122,7 +122,7 @@ std::vector<bool> latency_pattern(GPU_VBD_FEM& f, const GatherTile& t) {
 
 void release_unused(GPU_VBD_FEM& f) {
     if (!f.cached()) f.ju.reset();
-    const int keep = f.tasks() ? (f.tile.inc > 0 ? f.tile.inc : max_task_inc(f.element)) : -1;
+    const int keep = f.tasks() ? f.fit_inc(f.tile.inc) : -1;
     for (auto it = f.thr_tables.begin(); it != f.thr_tables.end();)
         it = it->first == keep ? std::next(it) : f.thr_tables.erase(it);
 }
@@ -153,9 +153,10 @@ std::vector<GatherTile> tile_candidates(Element el, bool vbd, bool tasks_ok) {
     if (!vbd) return out;
     for (int L : {4, 8, 16, 32, 64}) out.push_back(GatherTile{L, 1, PASS_CACHE});
     if (!tasks_ok) return out;
-    const int maxi = max_task_inc(el);
+    const int maxi = el == Hexa27 ? kQ2FusedIncPerBlock : max_task_inc(el);
     std::vector<int> incs{maxi, maxi / 2, maxi / 4};
-    if (el == Hexa27) incs.push_back(kQ2PencilIncPerBlock);   // the pencil form of the throughput pass
+    if (el == Hexa27)   // the pencil form of the throughput pass
+        for (int inc : {kQ2PencilIncLatency, 21, kQ2PencilIncPerBlock, kQ2PencilMaxInc}) incs.push_back(inc);
     for (int inc : incs)
         for (int64_t lat : {int64_t{0}, int64_t{256}, kLatencyBlocks, int64_t{2048},
                             std::numeric_limits<int64_t>::max()}) {

@@ -140,13 +140,13 @@ def test_vbd_q2_element_tasks(engine, curve):
 @pytest.mark.parametrize("curve", [0.0, 0.02])
 def test_vbd_q2_pencil_tasks(engine, curve):
     """Q2 throughput form with pencil tasks (9 lanes x 3 points, blocks of 28
-    incidences) on every colour (no latency form): the same steps as the
-    27-lane throughput form and the oracle's, on straight (per-element
-    records) and curved (per-point records) meshes."""
+    or 14 incidences) on every colour (no latency form): the same steps as the
+    27-lane throughput form (blocks of 32) and the oracle's, on straight
+    (per-element records) and curved (per-point records) meshes."""
     import octopusengine_amd as oe
 
     res = {}
-    for inc in (32, 28):
+    for inc in (32, 28, 14):
         g, o, _, _ = make_pair(engine, 9, (4, 2, 2), curve=curve)
         g.set_tile_ex(64, 1, oe.PASS_TASKS, inc_per_block=inc, latency_blocks=0)
         assert g.get_tile_ex()["inc_per_block"] == inc
@@ -157,9 +157,10 @@ def test_vbd_q2_pencil_tasks(engine, curve):
         err = max_err(res[inc], o.positions())
         print(f"Q2 curve={curve} throughput form, {inc} incidences per block: 3-step err={err:.3e}", flush=True)
         assert err <= TOL1
-    pair = max_err(res[32], res[28])
-    print(f"Q2 pencil vs 27-lane form {pair:.3e}", flush=True)
-    assert pair <= TOL1 / 10
+    for inc in (28, 14):
+        pair = max_err(res[32], res[inc])
+        print(f"Q2 pencil ({inc} per block) vs 27-lane form {pair:.3e}", flush=True)
+        assert pair <= TOL1 / 10
 
 
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
