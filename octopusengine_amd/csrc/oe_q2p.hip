@@ -47,13 +47,22 @@ constexpr PencilTables make_pencil_tables() {
 }
 __constant__ PencilTables kPencil = make_pencil_tables();
 
-// A task group's LDS (243 floats): the node values, then (once stage A has
-// read them) the lanes' sums; T and Tz.
+// A task group's LDS: the node values, then (once stage A has read them) the
+// lanes' sums; T and Tz.  Components are outermost, so the 9 lanes of a group
+// touch 9 consecutive words in every stage (or 3, broadcast), and groups sit
+// 265 words apart (9 mod 32), so a half-wave's 32 lanes (3.5 groups) fall in
+// distinct banks.  The sums use a row stride of 10 for the same reason.  On
+// the 256-scene batch the layout with components innermost and 243-word
+// groups took 764 bank-conflict cycles per wave.
+constexpr int kSumStride = 10;
 struct GroupLDS {
-    float r1[81];   // stage 0: d[(a * 9 + b * 3 + c) * 3 + comp]; sum: s[m * 9 + v]
-    float t[81];    // T_ab(k) at [(a * 9 + b * 3 + k) * 3 + comp]
+    float r1[89];   // stage 0: d[comp * 27 + a * 9 + b * 3 + c]; sum: s[m * kSumStride + v]
+    float t[81];    // T_ab(k) at [comp * 27 + a * 9 + b * 3 + k]
     float tz[81];   // Tz_ab(k)
+    float pad[14];
 };
+static_assert(sizeof(GroupLDS) == 265 * sizeof(float) && 265 % 32 == 9, "group stride 9 mod 32 banks");
+static_assert(8 * kSumStride + 9 <= 89, "the sums fit the first region");
 
 // Waiting for the wave's own LDS operations orders them for every lane of the
 // wave (a wave's LDS operations complete in order); the compiler must not move
@@ -113,10 +122,8 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
             for (int a = 0; a < 3; ++a) {
                 float d[3];
                 rel_node(u[a], ref, ps.lo_unit, d);
-                float* o = &sh.r1[(a * 9 + m) * 3];
-                o[0] = d[0];
-                o[1] = d[1];
-                o[2] = d[2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sh.r1[c * 27 + a * 9 + m] = d[c];
             }
         }
     }
@@ -127,15 +134,11 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
         const float dc[3] = {d_at(B, 0, p2), d_at(B, 1, p2), d_at(B, 2, p2)};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float* x0 = &sh.r1[(a * 9 + p1 * 3 + 0) * 3];
-            const float* x1 = &sh.r1[(a * 9 + p1 * 3 + 1) * 3];
-            const float* x2 = &sh.r1[(a * 9 + p1 * 3 + 2) * 3];
-            float* t = &sh.t[(a * 9 + m) * 3];
-            float* tz = &sh.tz[(a * 9 + m) * 3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                t[c] = lc[0] * x0[c] + lc[1] * x1[c] + lc[2] * x2[c];
-                tz[c] = dc[0] * x0[c] + dc[1] * x1[c] + dc[2] * x2[c];
+                const float* x = &sh.r1[c * 27 + a * 9 + p1 * 3];
+                sh.t[c * 27 + a * 9 + m] = lc[0] * x[0] + lc[1] * x[1] + lc[2] * x[2];
+                sh.tz[c * 27 + a * 9 + m] = dc[0] * x[0] + dc[1] * x[1] + dc[2] * x[2];
             }
         }
     }
@@ -147,17 +150,13 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
         const float db[3] = {d_at(B, 0, p1), d_at(B, 1, p1), d_at(B, 2, p1)};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float* t0 = &sh.t[(a * 9 + 0 * 3 + p2) * 3];
-            const float* t1 = &sh.t[(a * 9 + 1 * 3 + p2) * 3];
-            const float* t2 = &sh.t[(a * 9 + 2 * 3 + p2) * 3];
-            const float* z0 = &sh.tz[(a * 9 + 0 * 3 + p2) * 3];
-            const float* z1 = &sh.tz[(a * 9 + 1 * 3 + p2) * 3];
-            const float* z2 = &sh.tz[(a * 9 + 2 * 3 + p2) * 3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                X[a][c] = lb[0] * t0[c] + lb[1] * t1[c] + lb[2] * t2[c];
-                Xy[a][c] = db[0] * t0[c] + db[1] * t1[c] + db[2] * t2[c];
-                Xz[a][c] = lb[0] * z0[c] + lb[1] * z1[c] + lb[2] * z2[c];
+                const float* t = &sh.t[c * 27 + a * 9 + p2];
+                const float* z = &sh.tz[c * 27 + a * 9 + p2];
+                X[a][c] = lb[0] * t[0] + lb[1] * t[3] + lb[2] * t[6];
+                Xy[a][c] = db[0] * t[0] + db[1] * t[3] + db[2] * t[6];
+                Xz[a][c] = lb[0] * z[0] + lb[1] * z[3] + lb[2] * z[6];
             }
         }
     }
@@ -204,13 +203,13 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
     // sum over the group's 9 lanes: stage A has read r1, so it holds the sums now
     if (act) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) sh.r1[m * 9 + v] = acc.v[v];
+        for (int v = 0; v < 9; ++v) sh.r1[m * kSumStride + v] = acc.v[v];
     }
     wave_sync();
     if (act) {
         float s = 0.f;
 #pragma unroll
-        for (int l = 0; l < 9; ++l) s += sh.r1[l * 9 + m];
+        for (int l = 0; l < 9; ++l) s += sh.r1[l * kSumStride + m];
         part[task][m] = s;
     }
     wave_sync();   // the next round's stage 0 rewrites r1
@@ -219,15 +218,17 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
 __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fem, const OwnersView& own,
                                              const MaterialView& mt, const q2::Basis1D& basis, const float4* y,
                                              float damping, float dt, const int4 blk) {
-    __shared__ GroupLDS sh[kGroups + 1];   // the last one serves lane 63, which stores nothing
+    __shared__ GroupLDS sh[kGroups];
     __shared__ float part[kQ2PencilMaxInc][9];
     const int lane = static_cast<int>(threadIdx.x);
     const int g = lane / 9, m = lane - 9 * g;
+    // lane 63 (g = 7, m = 0) stores nothing; its stage-B reads use group 6's rows
+    GroupLDS& mine = sh[g < kGroups ? g : kGroups - 1];
     // the weights of the lane's points, the same for every task
     const float wq_i[3] = {gload(fem.wq, kPencil.node[m]), gload(fem.wq, kPencil.node[9 + m]),
                            gload(fem.wq, kPencil.node[18 + m])};
     for (int round = 0; round < kRounds && round * kGroups < blk.w; ++round)
-        pencil_round(ps, fem, own.own, mt, basis, wq_i, blk.z, blk.w, round, g, m, sh[g], part);
+        pencil_round(ps, fem, own.own, mt, basis, wq_i, blk.z, blk.w, round, g, m, mine, part);
     if (lane < blk.y) solve_slot_direct(ps, own, y, damping, dt, blk.x + lane, part, blk.z);
 }
 
