@@ -59,7 +59,8 @@ struct GroupLDS {
     float r1[89];   // stage 0: d[comp * 27 + a * 9 + b * 3 + c]; sum: s[m * kSumStride + v]
     float t[81];    // T_ab(k) at [comp * 27 + a * 9 + b * 3 + k]
     float tz[81];   // Tz_ab(k)
-    float pad[14];
+    float rec[9];   // an affine element's JX_inv, word m stored by lane m
+    float pad[5];
 };
 static_assert(sizeof(GroupLDS) == 265 * sizeof(float) && 265 % 32 == 9, "group stride 9 mod 32 banks");
 static_assert(8 * kSumStride + 9 <= 89, "the sums fit the first region");
@@ -78,36 +79,49 @@ __device__ __forceinline__ float l_at(const q2::Basis1D& B, int a, int i) { retu
 __device__ __forceinline__ float d_at(const q2::Basis1D& B, int a, int i) { return B.D[a][i]; }
 __device__ __forceinline__ float sel3(int k, float v0, float v1, float v2) { return k == 0 ? v0 : (k == 1 ? v1 : v2); }
 
+// The node ids of lane m's nodes (a, b, c), a = 0..2, for the group's task.
+// own_l: lane l holds the block's owners entry l (a block has at most 35).
+// Every lane runs the shuffle; only the task's lanes load.
+__device__ __forceinline__ void pencil_nodes(const FemView& f, int own_l, int task, int n_inc, int g, int m,
+                                             int (&nid)[3]) {
+    const int pk = __shfl(own_l, task);
+    if (g < kGroups && task < n_inc) {
+        const size_t base = static_cast<size_t>(pk >> 5) * 27;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) nid[a] = gload(f.topo, base + kPencil.node[a * 9 + m]);
+    }
+}
+
 // One round of the block's tasks: task = round * 7 + g for the lane's group g.
-// wq_i: the weights of the lane's points (i, j, k), i = 0..2.
-__device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f, const int* __restrict__ own,
+// wq_i: the weights of the lane's points (i, j, k), i = 0..2.  nid: this
+// round's node ids on entry, the next round's on return (loaded while this
+// round computes, so a round waits on one dependent load, not three).
+__device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f, int own_l, int (&nid)[3],
                                              const MaterialView& mt, const q2::Basis1D& B, const float (&wq_i)[3],
-                                             int first_own, int n_inc, int round, int g, int m, GroupLDS& sh,
-                                             float (*part)[9]) {
+                                             int n_inc, int round, int g, int m, GroupLDS& sh, float (*part)[9]) {
     const int task = round * kGroups + g;
     const bool act = g < kGroups && task < n_inc;
     const int p1 = m / 3, p2 = m - 3 * p1;   // stage 0 / A: (b, c) and (b, k); stage B / C: (j, k)
+    const int pk = __shfl(own_l, task);
     int eid = 0, r = 0;
     float4 u[3];
     if (act) {
-        const int pk = gload(own, first_own + task);
         eid = pk >> 5;
         r = pk & 31;
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            u[a] = gload(static_cast<const float4*>(ps.p),
-                         gload(f.topo, static_cast<size_t>(eid) * 27 + kPencil.node[a * 9 + m]));
+        for (int a = 0; a < 3; ++a) u[a] = gload(static_cast<const float4*>(ps.p), nid[a]);
     } else {
 #pragma unroll
         for (int a = 0; a < 3; ++a) u[a] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     // the element's record: JX_inv and |det J| (per point when not affine:
-    // point (0, j, k) here, the others below)
-    float JXi[9], V0 = 0.f, detB = 0.f;
+    // point (0, j, k) here, the others below).  An affine record is the same
+    // for the group's 9 lanes: lane m loads word m and passes it on through
+    // LDS, so the texture path returns 2 words per lane instead of 10.
+    float JXi[9], V0 = 0.f, detB = 0.f, rec_m = 0.f;
     if (act) {
         if (f.affine) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) JXi[i] = gload(f.jxv, static_cast<size_t>(eid) * 10 + i);
+            rec_m = gload(f.jxv, static_cast<size_t>(eid) * 10 + m);
             detB = gload(f.jxv, static_cast<size_t>(eid) * 10 + 9);
         } else {
             load_element_record(f, eid, kPencil.node[m], JXi, V0);
@@ -125,8 +139,10 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
 #pragma unroll
                 for (int c = 0; c < 3; ++c) sh.r1[c * 27 + a * 9 + m] = d[c];
             }
+            if (f.affine) sh.rec[m] = rec_m;
         }
     }
+    pencil_nodes(f, own_l, task + kGroups, n_inc, g, m, nid);
     wave_sync();
     // stage A: lane (b, k) = (p1, p2)
     if (act) {
@@ -167,6 +183,10 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
     Acc9 acc;
     acc_zero(acc);
     if (act) {
+        if (f.affine) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) JXi[i] = sh.rec[i];
+        }
         const int tr = kPencil.place[r];
         const int ra = tr / 9, rb = tr / 3 % 3, rc = tr % 3;
         const float lrb = sel3(rb, l_at(B, 0, p1), l_at(B, 1, p1), l_at(B, 2, p1));
@@ -227,8 +247,11 @@ __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fe
     // the weights of the lane's points, the same for every task
     const float wq_i[3] = {gload(fem.wq, kPencil.node[m]), gload(fem.wq, kPencil.node[9 + m]),
                            gload(fem.wq, kPencil.node[18 + m])};
+    const int own_l = lane < blk.w ? gload(own.own, blk.z + lane) : 0;
+    int nid[3] = {0, 0, 0};
+    pencil_nodes(fem, own_l, g, blk.w, g, m, nid);
     for (int round = 0; round < kRounds && round * kGroups < blk.w; ++round)
-        pencil_round(ps, fem, own.own, mt, basis, wq_i, blk.z, blk.w, round, g, m, mine, part);
+        pencil_round(ps, fem, own_l, nid, mt, basis, wq_i, blk.w, round, g, m, mine, part);
     if (lane < blk.y) solve_slot_direct(ps, own, y, damping, dt, blk.x + lane, part, blk.z);
 }
 
