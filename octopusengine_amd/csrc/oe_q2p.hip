@@ -77,6 +77,16 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ float l_at(const q2::Basis1D& B, int a, int i) { return B.L[a][i]; }
 __device__ __forceinline__ float d_at(const q2::Basis1D& B, int a, int i) { return B.D[a][i]; }
+
+// The 1D factors by Gauss point (row p: l_0, l_1, l_2 at p; the same for l')
+// and the tensor place of each node, copied to LDS once per block.  A lane
+// indexes them by its own (j, k) and node: from LDS these reads neither go
+// through the texture path nor wait behind the vector-memory loads in flight.
+struct PencilConst {
+    float4 l[3];
+    float4 d[3];
+    int place[27];
+};
 __device__ __forceinline__ float sel3(int k, float v0, float v1, float v2) { return k == 0 ? v0 : (k == 1 ? v1 : v2); }
 
 // The node ids of lane m's nodes (a, b, c), a = 0..2, for the group's task.
@@ -97,7 +107,8 @@ __device__ __forceinline__ void pencil_nodes(const FemView& f, int own_l, int ta
 // round's node ids on entry, the next round's on return (loaded while this
 // round computes, so a round waits on one dependent load, not three).
 __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f, int own_l, int (&nid)[3],
-                                             const MaterialView& mt, const q2::Basis1D& B, const float (&wq_i)[3],
+                                             const MaterialView& mt, const q2::Basis1D& B, const PencilConst& pc,
+                                             const float (&wq_i)[3],
                                              int n_inc, int round, int g, int m, GroupLDS& sh, float (*part)[9]) {
     const int task = round * kGroups + g;
     const bool act = g < kGroups && task < n_inc;
@@ -146,15 +157,25 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
     wave_sync();
     // stage A: lane (b, k) = (p1, p2)
     if (act) {
-        const float lc[3] = {l_at(B, 0, p2), l_at(B, 1, p2), l_at(B, 2, p2)};
-        const float dc[3] = {d_at(B, 0, p2), d_at(B, 1, p2), d_at(B, 2, p2)};
+        const float4 L = pc.l[p2], D = pc.d[p2];
+        const float lc[3] = {L.x, L.y, L.z};
+        const float dc[3] = {D.x, D.y, D.z};
+        // all 27 reads first: the compiler cannot tell that the writes to T
+        // miss them, and would otherwise wait on LDS after every write
+        float x[27];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) x[(a * 3 + c) * 3 + cc] = sh.r1[c * 27 + a * 9 + p1 * 3 + cc];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float* x = &sh.r1[c * 27 + a * 9 + p1 * 3];
-                sh.t[c * 27 + a * 9 + m] = lc[0] * x[0] + lc[1] * x[1] + lc[2] * x[2];
-                sh.tz[c * 27 + a * 9 + m] = dc[0] * x[0] + dc[1] * x[1] + dc[2] * x[2];
+                const float* xv = &x[(a * 3 + c) * 3];
+                sh.t[c * 27 + a * 9 + m] = lc[0] * xv[0] + lc[1] * xv[1] + lc[2] * xv[2];
+                sh.tz[c * 27 + a * 9 + m] = dc[0] * xv[0] + dc[1] * xv[1] + dc[2] * xv[2];
             }
         }
     }
@@ -162,8 +183,9 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
     // stage B: lane (j, k) = (p1, p2); X, Xy, Xz for a = 0..2 in registers
     float X[3][3], Xy[3][3], Xz[3][3];
     {
-        const float lb[3] = {l_at(B, 0, p1), l_at(B, 1, p1), l_at(B, 2, p1)};
-        const float db[3] = {d_at(B, 0, p1), d_at(B, 1, p1), d_at(B, 2, p1)};
+        const float4 L = pc.l[p1], D = pc.d[p1];
+        const float lb[3] = {L.x, L.y, L.z};
+        const float db[3] = {D.x, D.y, D.z};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -187,12 +209,13 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
 #pragma unroll
             for (int i = 0; i < 9; ++i) JXi[i] = sh.rec[i];
         }
-        const int tr = kPencil.place[r];
+        const int tr = pc.place[r];
         const int ra = tr / 9, rb = tr / 3 % 3, rc = tr % 3;
-        const float lrb = sel3(rb, l_at(B, 0, p1), l_at(B, 1, p1), l_at(B, 2, p1));
-        const float drb = sel3(rb, d_at(B, 0, p1), d_at(B, 1, p1), d_at(B, 2, p1));
-        const float lrc = sel3(rc, l_at(B, 0, p2), l_at(B, 1, p2), l_at(B, 2, p2));
-        const float drc = sel3(rc, d_at(B, 0, p2), d_at(B, 1, p2), d_at(B, 2, p2));
+        const float4 L1 = pc.l[p1], D1 = pc.d[p1], L2 = pc.l[p2], D2 = pc.d[p2];
+        const float lrb = sel3(rb, L1.x, L1.y, L1.z);
+        const float drb = sel3(rb, D1.x, D1.y, D1.z);
+        const float lrc = sel3(rc, L2.x, L2.y, L2.z);
+        const float drc = sel3(rc, D2.x, D2.y, D2.z);
         const float ll = lrb * lrc, dl = drb * lrc, ld = lrb * drc;
         // one point at a time (unrolled, the three points' material terms
         // interleave and hold registers that cost a wave per SIMD)
@@ -240,6 +263,7 @@ __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fe
                                              float damping, float dt, const int4 blk) {
     __shared__ GroupLDS sh[kGroups];
     __shared__ float part[kQ2PencilMaxInc][9];
+    __shared__ PencilConst pc;
     const int lane = static_cast<int>(threadIdx.x);
     const int g = lane / 9, m = lane - 9 * g;
     // lane 63 (g = 7, m = 0) stores nothing; its stage-B reads use group 6's rows
@@ -247,11 +271,16 @@ __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fe
     // the weights of the lane's points, the same for every task
     const float wq_i[3] = {gload(fem.wq, kPencil.node[m]), gload(fem.wq, kPencil.node[9 + m]),
                            gload(fem.wq, kPencil.node[18 + m])};
+    if (lane < 3) {
+        pc.l[lane] = make_float4(l_at(basis, 0, lane), l_at(basis, 1, lane), l_at(basis, 2, lane), 0.f);
+        pc.d[lane] = make_float4(d_at(basis, 0, lane), d_at(basis, 1, lane), d_at(basis, 2, lane), 0.f);
+    }
+    if (lane < 27) pc.place[lane] = kPencil.place[lane];
     const int own_l = lane < blk.w ? gload(own.own, blk.z + lane) : 0;
     int nid[3] = {0, 0, 0};
     pencil_nodes(fem, own_l, g, blk.w, g, m, nid);
     for (int round = 0; round < kRounds && round * kGroups < blk.w; ++round)
-        pencil_round(ps, fem, own_l, nid, mt, basis, wq_i, blk.w, round, g, m, mine, part);
+        pencil_round(ps, fem, own_l, nid, mt, basis, pc, wq_i, blk.w, round, g, m, mine, part);
     if (lane < blk.y) solve_slot_direct(ps, own, y, damping, dt, blk.x + lane, part, blk.z);
 }
 
