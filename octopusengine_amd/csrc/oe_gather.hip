@@ -348,10 +348,37 @@ __device__ __forceinline__ void jx_from_grad(const float (&xs)[S][3], const Grad
     }
 }
 
+// In the throughput form of the fused blocks, shape-derivative tables of at
+// most kDnLdsFloats floats (P1, P2, Q1) are copied to LDS, one copy per wave,
+// at the start of a block: a lane's dN_r at each point is then an LDS read,
+// not a device-memory load that the point loop waits on (P2 batch: 84.9
+// against 89.1 us per colour pass).  The latency form keeps the loads: it
+// issues the vertex state's loads first, and vector-memory loads complete in
+// order, so the copy's LDS write would wait for them (single P2 scene: 1.22
+// against 1.19 ms per step).
+constexpr int kDnLdsFloats = 256;
+template <int S, int Q, bool LAT>
+constexpr bool dn_in_lds() { return !LAT && S * Q * 3 <= kDnLdsFloats; }
+template <int S, int Q, bool LAT>
+constexpr int dn_lds_size() { return dn_in_lds<S, Q, LAT>() ? S * Q * 3 : 1; }
+
+template <int S, int Q, bool DNL>
+__device__ __forceinline__ void dnr_at(const FemView& f, const float* dnw, int q, int r, float dnr[3]) {
+    if constexpr (DNL) {
+        const float* d = dnw + (q * S + r) * 3;
+        dnr[0] = d[0];
+        dnr[1] = d[1];
+        dnr[2] = d[2];
+    } else {
+        load_dnr(f.dN, (q * S + r) * 3, dnr);
+    }
+}
+
 // Partial (force, Hessian) of local node r of element eid over its Q points.
-template <int S, int Q, class Grad>
+// dnw: the wave's LDS copy of the shape-derivative table (DNL only).
+template <int S, int Q, bool DNL, class Grad>
 __device__ __forceinline__ void elem_partial(const FemView& f, const MaterialView& mt, const Grad& grad, int eid,
-                                             int r, const float (&xs)[S][3], Acc9& acc) {
+                                             int r, const float (&xs)[S][3], const float* dnw, Acc9& acc) {
     acc_zero(acc);
     if (f.affine) {
         // one record per element: JX_inv and |det B|, V_q = |det B| w_q
@@ -364,7 +391,7 @@ __device__ __forceinline__ void elem_partial(const FemView& f, const MaterialVie
         for (int q = 0; q < Q; ++q) {
             float Ju[9], dnr[3];
             jx_from_grad<S>(xs, grad, q * S * 3, Ju);
-            load_dnr(f.dN, (q * S + r) * 3, dnr);
+            dnr_at<S, Q, DNL>(f, dnw, q, r, dnr);
             nh_contrib(Ju, JXi, detB * gload(f.wq, q), dnr, mt, acc);
         }
     } else {
@@ -373,23 +400,45 @@ __device__ __forceinline__ void elem_partial(const FemView& f, const MaterialVie
             float Ju[9], JXi[9], V, dnr[3];
             load_element_record(f, eid, q, JXi, V);
             jx_from_grad<S>(xs, grad, q * S * 3, Ju);
-            load_dnr(f.dN, (q * S + r) * 3, dnr);
+            dnr_at<S, Q, DNL>(f, dnw, q, r, dnr);
             nh_contrib(Ju, JXi, V, dnr, mt, acc);
         }
     }
 }
 
 // Phase 1 of a fused block: the partials of the block's incidences into LDS.
-template <int S, int Q, class Grad>
+template <int S, int Q, bool DNL, class Grad>
 __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f, const OwnersView& own,
                                              const MaterialView& mt, const Grad& grad, const int4 blk,
-                                             float (*part)[9]) {
+                                             float (*part)[9], float* dnw) {
     const int first_own = blk.z, n_inc = blk.w;
     const int t0 = threadIdx.x;
+    // the table copy's loads go first and its LDS writes after the first
+    // owners load is issued, so waiting for them overlaps that load; every
+    // lane of the wave writes its share before any lane leaves
+    constexpr int kDnPerLane = DNL ? (S * Q * 3 + 63) / 64 : 0;
+    float dn_stage[kDnPerLane > 0 ? kDnPerLane : 1];
+    const int lane = t0 & 63;
+    if constexpr (kDnPerLane > 0) {
+#pragma unroll
+        for (int j = 0; j < kDnPerLane; ++j)
+            if (lane + 64 * j < S * Q * 3) dn_stage[j] = gload(f.dN, lane + 64 * j);
+    }
+    const int pk0 = t0 < n_inc ? gload(own.own, first_own + t0) : 0;
+    if constexpr (kDnPerLane > 0) {
+#pragma unroll
+        for (int j = 0; j < kDnPerLane; ++j)
+            if (lane + 64 * j < S * Q * 3) dnw[lane + 64 * j] = dn_stage[j];
+        // a wave's LDS operations complete in order; keep the compiler from
+        // moving the table reads above the writes
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
     if (t0 >= n_inc) return;
 #pragma unroll 1
     for (int t = t0; t < n_inc; t += kFusedThreads) {
-        const int pk = gload(own.own, first_own + t);
+        const int pk = t == t0 ? pk0 : gload(own.own, first_own + t);
         float4 h[S];
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -399,7 +448,7 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
         for (int k = 0; k < S; ++k) rel_node(h[k], h[0], ps.lo_unit, xs[k]);
         const int eid = pk >> 5, r = pk & 31;
         Acc9 acc;
-        elem_partial<S, Q>(f, mt, grad, eid, r, xs, acc);
+        elem_partial<S, Q, DNL>(f, mt, grad, eid, r, xs, dnw, acc);
 #pragma unroll
         for (int i = 0; i < 9; ++i) part[t][i] = acc.v[i];
     }
@@ -408,17 +457,17 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
 template <int S, int Q, bool LAT, class Grad>
 __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, const OwnersView& own,
                                             const MaterialView& mt, const Grad& grad, const float4* y, float damping,
-                                            float dt, const int4 blk, float (*part)[9]) {
+                                            float dt, const int4 blk, float (*part)[9], float* dnw) {
     const int first_slot = blk.x, n_slots = blk.y, first_own = blk.z;
     const int t0 = threadIdx.x;
     if constexpr (LAT) {
         SlotState st{};
         if (t0 < n_slots) st = load_slot_state(ps, own, y, first_slot + t0);
-        fused_phase1<S, Q>(ps, f, own, mt, grad, blk, part);
+        fused_phase1<S, Q, dn_in_lds<S, Q, LAT>()>(ps, f, own, mt, grad, blk, part, dnw);
         __syncthreads();
         if (t0 < n_slots && st.free != 0) solve_slot_from_lds(ps, st, part, first_own, damping, dt);
     } else {
-        fused_phase1<S, Q>(ps, f, own, mt, grad, blk, part);
+        fused_phase1<S, Q, dn_in_lds<S, Q, LAT>()>(ps, f, own, mt, grad, blk, part, dnw);
         __syncthreads();
         // phase 2: one thread per slot, partials in the owners order, local solve
         if (t0 >= n_slots) return;
@@ -429,23 +478,27 @@ __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, 
 template <int S, int Q, bool LAT>
 __device__ __forceinline__ void fused_dispatch(const PSView& ps, const FemView& f, const OwnersView& own,
                                                const MaterialView& mt, const ShapeGradArg<S, Q>& g, const float4* y,
-                                               float damping, float dt, const int4 blk, float (*part)[9]) {
+                                               float damping, float dt, const int4 blk, float (*part)[9],
+                                               float* dnw) {
     if constexpr (grad_by_value<S, Q>())
-        fused_block<S, Q, LAT>(ps, f, own, mt, GradByValue<S, Q>{g}, y, damping, dt, blk, part);
+        fused_block<S, Q, LAT>(ps, f, own, mt, GradByValue<S, Q>{g}, y, damping, dt, blk, part, dnw);
     else
-        fused_block<S, Q, LAT>(ps, f, own, mt, GradInMemory{f.dN}, y, damping, dt, blk, part);
+        fused_block<S, Q, LAT>(ps, f, own, mt, GradInMemory{f.dN}, y, damping, dt, blk, part, dnw);
 }
 
 template <int S, int Q, bool LAT>
 __global__ __launch_bounds__(kFusedThreads) void k_fused_tasks(FusedTaskArgs a, ShapeGradArg<S, Q> g) {
     __shared__ float part[LAT ? kFusedIncPerBlockLat : kFusedIncPerBlock][9];
+    __shared__ float dn_lds[kFusedThreads / 64][dn_lds_size<S, Q, LAT>()];
     const int4 blk = gload(a.blocks, a.first_block + xcd_block(blockIdx.x, gridDim.x));
-    fused_dispatch<S, Q, LAT>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part);
+    fused_dispatch<S, Q, LAT>(a.ps, a.fem, a.own, a.mt, g, a.y, a.damping, a.dt, blk, part,
+                              dn_lds[threadIdx.x >> 6]);
 }
 
 template <int S, int Q, bool LAT>
 __global__ __launch_bounds__(kFusedThreads) void k_batch_fused_tasks(BatchFusedArgs b, ShapeGradArg<S, Q> g) {
     __shared__ float part[LAT ? kFusedIncPerBlockLat : kFusedIncPerBlock][9];
+    __shared__ float dn_lds[kFusedThreads / 64][dn_lds_size<S, Q, LAT>()];
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const BatchSeg seg = find_seg(b.segs, b.seg_first, b.seg_count, bid);
     const SceneDev& sd = b.scenes[seg.scene];
@@ -458,7 +511,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_batch_fused_tasks(BatchFusedA
     const float4* y = sd.y;
     const float damping = sd.damping;
     const int4 blk = gload(LAT ? sd.fblocks_lat : sd.fblocks, seg.first + (bid - seg.block_begin));
-    fused_dispatch<S, Q, LAT>(ps, fem, own, mt, g, y, damping, b.dt, blk, part);
+    fused_dispatch<S, Q, LAT>(ps, fem, own, mt, g, y, damping, b.dt, blk, part, dn_lds[threadIdx.x >> 6]);
 }
 
 #endif
