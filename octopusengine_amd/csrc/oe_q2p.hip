@@ -90,16 +90,13 @@ struct PencilConst {
 __device__ __forceinline__ float sel3(int k, float v0, float v1, float v2) { return k == 0 ? v0 : (k == 1 ? v1 : v2); }
 
 // The node ids of lane m's nodes (a, b, c), a = 0..2, for the group's task.
-// own_l: lane l holds the block's owners entry l (a block has at most 35).
-// Every lane runs the shuffle; only the task's lanes load.
-__device__ __forceinline__ void pencil_nodes(const FemView& f, int own_l, int task, int n_inc, int g, int m,
-                                             int (&nid)[3]) {
+// own_l: lane l holds the block's owners entry l (a block has at most 35
+// incidences; lanes past them hold 0, element 0).  task < 43 for every lane.
+__device__ __forceinline__ void pencil_nodes(const FemView& f, int own_l, int task, int m, int (&nid)[3]) {
     const int pk = __shfl(own_l, task);
-    if (g < kGroups && task < n_inc) {
-        const size_t base = static_cast<size_t>(pk >> 5) * 27;
+    const size_t base = static_cast<size_t>(pk >> 5) * 27;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) nid[a] = gload(f.topo, base + kPencil.node[a * 9 + m]);
-    }
+    for (int a = 0; a < 3; ++a) nid[a] = gload(f.topo, base + kPencil.node[a * 9 + m]);
 }
 
 // One round of the block's tasks: task = round * 7 + g for the lane's group g.
@@ -111,38 +108,36 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
                                              const float (&wq_i)[3],
                                              int n_inc, int round, int g, int m, GroupLDS& sh, float (*part)[9]) {
     const int task = round * kGroups + g;
+    // Every lane computes; only the stores are guarded.  A lane without a task
+    // works on valid data: the owners entry of lane task is 0 (element 0,
+    // node 0) from the block's last incidence on, and nid holds earlier node
+    // ids or 0.  Lane 63 reads group 6's rows and must not write them.  (A
+    // lane that skipped the work would need its values zeroed on every round,
+    // and lane 63 never has a task.)
     const bool act = g < kGroups && task < n_inc;
+    const bool writer = g < kGroups;
     const int p1 = m / 3, p2 = m - 3 * p1;   // stage 0 / A: (b, c) and (b, k); stage B / C: (j, k)
     const int pk = __shfl(own_l, task);
-    int eid = 0, r = 0;
+    const int eid = pk >> 5, r = pk & 31;
     float4 u[3];
-    if (act) {
-        eid = pk >> 5;
-        r = pk & 31;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) u[a] = gload(static_cast<const float4*>(ps.p), nid[a]);
-    } else {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) u[a] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (int a = 0; a < 3; ++a) u[a] = gload(static_cast<const float4*>(ps.p), nid[a]);
     // the element's record: JX_inv and |det J| (per point when not affine:
     // point (0, j, k) here, the others below).  An affine record is the same
     // for the group's 9 lanes: lane m loads word m and passes it on through
     // LDS, so the texture path returns 2 words per lane instead of 10.
     float JXi[9], V0 = 0.f, detB = 0.f, rec_m = 0.f;
-    if (act) {
-        if (f.affine) {
-            rec_m = gload(f.jxv, static_cast<size_t>(eid) * 10 + m);
-            detB = gload(f.jxv, static_cast<size_t>(eid) * 10 + 9);
-        } else {
-            load_element_record(f, eid, kPencil.node[m], JXi, V0);
-        }
+    if (f.affine) {
+        rec_m = gload(f.jxv, static_cast<size_t>(eid) * 10 + m);
+        detB = gload(f.jxv, static_cast<size_t>(eid) * 10 + 9);
+    } else {
+        load_element_record(f, eid, kPencil.node[m], JXi, V0);
     }
     // stage 0: node 0 (tensor place 0) is the a = 0 node of lane m = 0 of the group
     {
         const int src = (g < kGroups ? g : 0) * 9;
         const float4 ref = make_float4(__shfl(u[0].x, src), __shfl(u[0].y, src), __shfl(u[0].z, src), 0.f);
-        if (act) {
+        if (writer) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 float d[3];
@@ -153,10 +148,10 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
             if (f.affine) sh.rec[m] = rec_m;
         }
     }
-    pencil_nodes(f, own_l, task + kGroups, n_inc, g, m, nid);
+    pencil_nodes(f, own_l, task + kGroups, m, nid);
     wave_sync();
     // stage A: lane (b, k) = (p1, p2)
-    if (act) {
+    if (writer) {
         const float4 L = pc.l[p2], D = pc.d[p2];
         const float lc[3] = {L.x, L.y, L.z};
         const float dc[3] = {D.x, D.y, D.z};
@@ -204,7 +199,7 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
     // dN_r(i, j, k) = (l'_ra(x_i) l_rb(y_j) l_rc(z_k), l_ra l'_rb l_rc, l_ra l_rb l'_rc).
     Acc9 acc;
     acc_zero(acc);
-    if (act) {
+    {
         if (f.affine) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) JXi[i] = sh.rec[i];
@@ -244,7 +239,7 @@ __device__ __forceinline__ void pencil_round(const PSView& ps, const FemView& f,
         }
     }
     // sum over the group's 9 lanes: stage A has read r1, so it holds the sums now
-    if (act) {
+    if (writer) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) sh.r1[m * kSumStride + v] = acc.v[v];
     }
@@ -277,8 +272,8 @@ __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fe
     }
     if (lane < 27) pc.place[lane] = kPencil.place[lane];
     const int own_l = lane < blk.w ? gload(own.own, blk.z + lane) : 0;
-    int nid[3] = {0, 0, 0};
-    pencil_nodes(fem, own_l, g, blk.w, g, m, nid);
+    int nid[3];
+    pencil_nodes(fem, own_l, g, m, nid);
     for (int round = 0; round < kRounds && round * kGroups < blk.w; ++round)
         pencil_round(ps, fem, own_l, nid, mt, basis, pc, wq_i, blk.w, round, g, m, mine, part);
     if (lane < blk.y) solve_slot_direct(ps, own, y, damping, dt, blk.x + lane, part, blk.z);
