@@ -460,6 +460,11 @@ def served_leg(args) -> dict:
     nv = np.array([s.n_vertices for s in scenes])
     srv = oe.Server(eng, max_batch=args.serve_max_batch, streams=args.serve_streams)
     slots = [srv.add_scene(s) for s in scenes]
+    if args.serve_q2_inc or args.serve_q2_lat_blocks >= 0:
+        q2 = srv.get_tile_ex(oe.HEXA27)
+        srv.set_tile_ex(oe.HEXA27, q2["lanes"], q2["qpl"], q2["mode"],
+                        args.serve_q2_inc or q2["inc_per_block"],
+                        args.serve_q2_lat_blocks if args.serve_q2_lat_blocks >= 0 else q2["latency_blocks"])
     for sl in slots:                          # warm-up: every scene once
         srv.submit(sl, DT, 1)
     srv.drain()
@@ -511,6 +516,7 @@ def served_leg(args) -> dict:
         "slowest_1pct_share_q2": float((els[slowest] == oe.HEXA27).mean()),
         "slowest_1pct_median_vertices": float(np.median(nvs[slowest])),
         "max_batch": args.serve_max_batch, "streams": args.serve_streams,
+        "q2_tile": srv.get_tile_ex(oe.HEXA27),
         "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "setup_s": setup_s,
         "device_bytes": int(eng.memory()[0]), "budget_bytes": int(eng.memory()[1]),
     }
@@ -550,8 +556,13 @@ def main():
     ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-q2-inc", type=int, default=0,
+                    help="incidences per block of the served Q2 passes (0: the server's default)")
+    ap.add_argument("--serve-q2-lat-blocks", type=int, default=-1,
+                    help="throughput-form blocks below which a served Q2 colour runs the latency form (-1: default)")
     ap.add_argument("--eager-only", action="store_true",
                     help="batched leg: only the eager timed region (no graph, latency or plain loop)")
+    ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
     ap.add_argument("--single-only", action="store_true",
                     help="run only the single-scene leg (P2 64x16x16) and print its figures (profiling)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -567,6 +578,10 @@ def main():
     if args.single_only:
         print(json.dumps(_finite_or_null({"plain_single_scene_p2_64x16x16": plain_single_leg(args, eng)})), flush=True)
         eng.close()
+        return
+    if args.serve_only:
+        eng.close()
+        print(json.dumps(_finite_or_null({"served": served_leg(args)})), flush=True)
         return
     bat = batched_leg(args, eng)
     progress(f"batched leg: {bat['ms_per_step']:.2f} ms per step")

@@ -120,9 +120,11 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
 oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* mode);
 /* The full colour-pass layout: the gather knobs above plus, for element
  * tasks, the incidences per block (the elements one workgroup stages in LDS;
- * 0 = the kernel's maximum, 256 for tetrahedra and 8-node hexahedra, 32 for
- * 27-node hexahedra) and the throughput-form block count below which a
- * colour runs in the kernels' latency form (default 512).  No counterpart in
+ * 0 = the kernel's maximum, 256 for tetrahedra and 8-node hexahedra, 35 for
+ * 27-node hexahedra, where 14, 21, 28 and 35 select the throughput kernel with
+ * 9-lane tasks and 8, 16 and 32 the one with 27-lane tasks) and the
+ * throughput-form block count below which a colour runs in the kernels'
+ * latency form (default 512).  No counterpart in
  * the reference (src/GPU/VBD/GPU_VBD_FEM.cu:619-650 fixes one layout per
  * kernel version). */
 typedef struct oe_tile {
@@ -238,6 +240,12 @@ oe_status oe_server_destroy(oe_server* sv);
  * residual, VTK) first wait for every request submitted for the scene;
  * oe_scene_destroy waits for them and releases the slot. */
 oe_status oe_server_add_scene(oe_server* sv, oe_scene* s, int32_t* slot);
+/* Layout of one element type's colour passes in every tick (default: the
+ * scene defaults, with 14 incidences per block for 27-node hexahedra).
+ * Refused (OE_ERR_STATE) while requests are pending or in flight; get returns
+ * the layout in use.  No counterpart in the reference. */
+oe_status oe_server_set_tile_ex(oe_server* sv, int32_t element, const oe_tile* t);
+oe_status oe_server_get_tile_ex(oe_server* sv, int32_t element, oe_tile* t);
 /* Thread-safe; records the arrival time of the request.  Requests of one
  * scene run in submission order. */
 oe_status oe_server_submit(oe_server* sv, int32_t slot, float dt, int32_t substeps, uint64_t* ticket);

@@ -576,6 +576,16 @@ class Server:
         L.check(L.lib.oe_server_submit(self._h, slot, dt, substeps, C.byref(t)))
         return t.value
 
+    def set_tile_ex(self, element: int, lanes=64, qpl=1, mode=PASS_TASKS, inc_per_block=0, latency_blocks=512):
+        """Layout of one element type's passes in every tick (refused while requests are in flight)."""
+        t = L.Tile(lanes, qpl, int(mode), inc_per_block, latency_blocks)
+        L.check(L.lib.oe_server_set_tile_ex(self._h, element, C.byref(t)))
+
+    def get_tile_ex(self, element: int) -> dict:
+        t = L.Tile()
+        L.check(L.lib.oe_server_get_tile_ex(self._h, element, C.byref(t)))
+        return t.as_dict()
+
     def drain(self):
         L.check(L.lib.oe_server_drain(self._h))
 

@@ -135,6 +135,8 @@ _SIGS = {
     "oe_server_create": (C.c_int32, [P, C.c_int32, C.c_int32, C.POINTER(P)]),
     "oe_server_destroy": (C.c_int32, [P]),
     "oe_server_add_scene": (C.c_int32, [P, P, i32p]),
+    "oe_server_set_tile_ex": (C.c_int32, [P, C.c_int32, C.POINTER(Tile)]),
+    "oe_server_get_tile_ex": (C.c_int32, [P, C.c_int32, C.POINTER(Tile)]),
     "oe_server_submit": (C.c_int32, [P, C.c_int32, C.c_float, C.c_int32, u64p]),
     "oe_server_drain": (C.c_int32, [P]),
     "oe_server_wait": (C.c_int32, [P, C.c_uint64]),

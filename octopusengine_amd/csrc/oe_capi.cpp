@@ -673,6 +673,21 @@ oe_status oe_server_add_scene(oe_server* sv, oe_scene* s, int32_t* slot) {
     });
 }
 
+oe_status oe_server_set_tile_ex(oe_server* sv, int32_t element, const oe_tile* t) {
+    return guard([&] {
+        OE_REQUIRE(sv && t && oe::element_supported(element), "bad argument");
+        check_tile_ex(element, t);
+        sv->server->set_tile(static_cast<oe::Element>(element), tile_from(t));
+    });
+}
+
+oe_status oe_server_get_tile_ex(oe_server* sv, int32_t element, oe_tile* t) {
+    return guard([&] {
+        OE_REQUIRE(sv && t && oe::element_supported(element), "bad argument");
+        tile_to(sv->server->tile(static_cast<oe::Element>(element)), t);
+    });
+}
+
 oe_status oe_server_submit(oe_server* sv, int32_t slot, float dt, int32_t substeps, uint64_t* ticket) {
     return guard([&] {
         OE_REQUIRE(sv, "null server");

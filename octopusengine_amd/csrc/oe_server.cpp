@@ -93,6 +93,30 @@ void Server::remove_scene(int slot) {
     s->server_slot = -1;
 }
 
+void Server::set_tile(Element e, GatherTile t) {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (!error_.empty()) throw std::runtime_error(error_);
+    if (!pending_.empty() || !inflight_.empty() || launching_ != 0)
+        throw std::logic_error("the server's layout changes only while no request is pending or in flight");
+    std::map<int, GatherTile> tiles = tiles_;
+    tiles[e] = t;
+    // every scene of the type first, so that a refused layout leaves the server as it was
+    std::vector<std::pair<size_t, SceneDev>> changed;
+    for (size_t i = 0; i < scenes_.size(); ++i)
+        if (scenes_[i] && scenes_[i]->mesh.element == e)
+            changed.emplace_back(i, make_scene_dev(*scenes_[i], flags_[i].get(), tiles));
+    tiles_ = std::move(tiles);
+    for (const auto& c : changed) host_table_[c.first] = c.second;
+    table_uploaded_ = 0;
+    grow_table();
+}
+
+GatherTile Server::tile(Element e) {
+    std::lock_guard<std::mutex> lk(mu_);
+    const auto it = tiles_.find(e);
+    return it != tiles_.end() ? it->second : default_tile(e);
+}
+
 void Server::grow_table() {
     if (table_.size() < host_table_.size()) {
         DBuf<SceneDev> bigger(&eng_->pool, std::max<size_t>(16, host_table_.size() * 2));

@@ -52,6 +52,9 @@ public:
     bool poll(uint64_t ticket);                              // true once the request is complete
     void wait_slot(int slot);                                // wait for every request of one scene
     void remove_scene(int slot);                             // wait_slot, then release the scene
+    // layout of one element type's passes; refused while requests are pending or in flight
+    void set_tile(Element e, GatherTile t);
+    GatherTile tile(Element e);
     struct Stats {
         int64_t completed = 0, vertex_updates = 0, batches = 0, max_batch_seen = 0;
         double p50_ms = 0, p99_ms = 0, max_ms = 0, mean_batch = 0;

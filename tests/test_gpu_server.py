@@ -63,6 +63,45 @@ def test_served_equals_alone(engine):
     srv.close()
 
 
+def test_server_tile_sets_the_served_layout(engine):
+    """A server's Q2 layout (incidences per block, latency threshold) applies
+    to every tick; pencil blocks of 14 and 28 incidences give the same steps."""
+    specs = [(oe.HEXA27, (6, 3, 2), 1e6, 0.45), (oe.HEXA27, (5, 2, 2), 3e5, 0.49)]
+    results = []
+    for inc in (14, 28):
+        scenes = [_scene(engine, sp) for sp in specs]
+        srv = oe.Server(engine, max_batch=2, streams=1)
+        slots = [srv.add_scene(sc) for sc in scenes]
+        t = srv.get_tile_ex(oe.HEXA27)
+        assert t["mode"] == oe.PASS_TASKS and t["inc_per_block"] == 14 and t["latency_blocks"] == 512
+        # latency_blocks 0: every colour in the throughput form, so the layout is used
+        srv.set_tile_ex(oe.HEXA27, t["lanes"], t["qpl"], t["mode"], inc, 0)
+        t = srv.get_tile_ex(oe.HEXA27)
+        assert t["inc_per_block"] == inc and t["latency_blocks"] == 0
+        for _ in range(3):
+            for sl in slots:
+                srv.submit(sl, 0.005, 1)
+        srv.drain()
+        results.append([sc.positions() for sc in scenes])
+        with pytest.raises(oe.OEError):
+            srv.set_tile_ex(oe.HEXA27, 64, 1, oe.PASS_TASKS, 36, 0)   # above the kernel's 35
+        with pytest.raises(oe.OEError):
+            srv.set_tile_ex(99, 64, 1, oe.PASS_TASKS, 14, 0)
+        srv.close()
+        for sc in scenes:
+            sc.close()
+    alone = [_scene(engine, sp) for sp in specs]
+    for sc in alone:
+        for _ in range(3):
+            sc.step(0.005, 1)
+    for k, (sp, sc) in enumerate(zip(specs, alone)):
+        d_inc = max_err(results[0][k], results[1][k])
+        d_alone = max_err(results[0][k], sc.positions())
+        print(f"served Q2 cells={sp[1]}: |inc14-inc28|={d_inc:.3e} |served-alone|={d_alone:.3e}", flush=True)
+        assert d_inc <= 1e-7
+        assert d_alone <= 1e-6
+
+
 def test_server_rejects_bad_requests(engine):
     s = _scene(engine, SPECS[0])
     srv = oe.Server(engine, max_batch=2, streams=1)
