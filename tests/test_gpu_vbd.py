@@ -139,14 +139,17 @@ def test_vbd_q2_element_tasks(engine, curve):
 
 @pytest.mark.parametrize("curve", [0.0, 0.02])
 def test_vbd_q2_pencil_tasks(engine, curve):
-    """Q2 throughput form with pencil tasks (9 lanes x 3 points, blocks of 28
-    or 14 incidences) on every colour (no latency form): the same steps as the
-    27-lane throughput form (blocks of 32) and the oracle's, on straight
-    (per-element records) and curved (per-point records) meshes."""
+    """Q2 throughput form with pencil tasks (9 lanes x 3 points, blocks of 14,
+    21, 28 or 35 incidences, so that a block's last round is full or partly
+    empty) on every colour (no latency form): the same steps as the 27-lane
+    throughput form (blocks of 32) and the oracle's, on straight (per-element
+    records) and curved (per-point records) meshes.  The pencil sizes compute
+    every task alike and sum each vertex in the owners order: their steps are
+    bit-identical."""
     import octopusengine_amd as oe
 
     res = {}
-    for inc in (32, 28, 14):
+    for inc in (32, 35, 28, 21, 14):
         g, o, _, _ = make_pair(engine, 9, (4, 2, 2), curve=curve)
         g.set_tile_ex(64, 1, oe.PASS_TASKS, inc_per_block=inc, latency_blocks=0)
         assert g.get_tile_ex()["inc_per_block"] == inc
@@ -157,10 +160,14 @@ def test_vbd_q2_pencil_tasks(engine, curve):
         err = max_err(res[inc], o.positions())
         print(f"Q2 curve={curve} throughput form, {inc} incidences per block: 3-step err={err:.3e}", flush=True)
         assert err <= TOL1
-    for inc in (28, 14):
+    for inc in (35, 28, 21, 14):
         pair = max_err(res[32], res[inc])
         print(f"Q2 pencil ({inc} per block) vs 27-lane form {pair:.3e}", flush=True)
         assert pair <= TOL1 / 10
+    for inc in (35, 21, 14):
+        pair = max_err(res[28], res[inc])
+        print(f"Q2 pencil {inc} vs 28 per block {pair:.3e}", flush=True)
+        assert pair == 0.0
 
 
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
