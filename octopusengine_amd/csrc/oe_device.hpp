@@ -288,6 +288,33 @@ struct SlotState {
     float m;
 };
 
+// The latency forms load a slot's state in two steps, each issued right after
+// a step of the block's own task chain (owners entry, then node ids), so that
+// no wait of that chain waits for the state: vector-memory loads complete in
+// order, and a wait for one load waits for every load issued before it.
+__device__ __forceinline__ void slot_state_level1(const OwnersView& own, int slot, SlotState& st) {
+    st.vid = gload(own.slot_vid, slot);
+    const int2 nbo = gload(own.slot_nbo, slot);
+    st.n_inc = nbo.x;
+    st.first_inc = nbo.y;
+}
+__device__ __forceinline__ void slot_state_level2(const PSView& ps, const float4* y, SlotState& st) {
+    st.free = gload(ps.mask, st.vid);
+    st.p = gload(static_cast<const float4*>(ps.p), st.vid);
+    st.last_p = gload(static_cast<const float4*>(ps.last_p), st.vid);
+    st.y = gload(y, st.vid);
+    st.m = gload(ps.m, st.vid);
+}
+// Where the state is first used: the empty asm takes the values as inputs and
+// outputs, so the compiler cannot convert them (and wait for their loads)
+// before this point.
+__device__ __forceinline__ void slot_state_settle(SlotState& st) {
+    asm volatile(""
+                 : "+v"(st.p.x), "+v"(st.p.y), "+v"(st.p.z), "+v"(st.p.w), "+v"(st.last_p.x), "+v"(st.last_p.y),
+                   "+v"(st.last_p.z), "+v"(st.last_p.w), "+v"(st.y.x), "+v"(st.y.y), "+v"(st.y.z), "+v"(st.y.w),
+                   "+v"(st.m), "+v"(st.free), "+v"(st.vid), "+v"(st.n_inc), "+v"(st.first_inc));
+}
+
 __device__ __forceinline__ SlotState load_slot_state(const PSView& ps, const OwnersView& own, const float4* y,
                                                      int slot) {
     SlotState st;

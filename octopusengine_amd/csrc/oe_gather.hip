@@ -454,6 +454,47 @@ __device__ __forceinline__ void fused_phase1(const PSView& ps, const FemView& f,
     }
 }
 
+// The latency form's phase 1: one incidence per thread, and the slot state's
+// loads issued by after_owner / after_topo right after the owners-entry and
+// node-id loads, so that no wait of the element chain covers them.  A wave past
+// the block's incidences returns at once (it holds no slot either).  The rest
+// is straight-line: a lane past the incidences works on the last one and
+// stores nothing (every vertex block has an incidence), so that no wait has to
+// cover the loads of another path, which would make the compiler wait for
+// every load in flight, the slot state's included.  (The throughput form keeps
+// the loop above: this form's prologue cost it 2%, 86.7 against 84.5 us per P2
+// batch pass on one GPU.)
+template <int S, int Q, class Grad, class AfterOwner, class AfterTopo>
+__device__ __forceinline__ void fused_phase1_lat(const PSView& ps, const FemView& f, const OwnersView& own,
+                                                 const MaterialView& mt, const Grad& grad, const int4 blk,
+                                                 float (*part)[9], AfterOwner&& after_owner, AfterTopo&& after_topo) {
+    static_assert(kFusedRuns == 1, "one incidence per thread");
+    const int first_own = blk.z, n_inc = blk.w;
+    const int t0 = threadIdx.x;
+    if (__builtin_amdgcn_readfirstlane(t0 & ~63) >= n_inc) return;
+    const bool has = t0 < n_inc;
+    const int pk = gload(own.own, first_own + (has ? t0 : n_inc - 1));
+    after_owner();
+    __builtin_amdgcn_sched_barrier(0);
+    int nid[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) nid[k] = gload(f.topo, static_cast<size_t>(pk >> 5) * S + k);
+    after_topo();
+    float4 h[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) h[k] = gload(static_cast<const float4*>(ps.p), nid[k]);
+    float xs[S][3];
+#pragma unroll
+    for (int k = 0; k < S; ++k) rel_node(h[k], h[0], ps.lo_unit, xs[k]);
+    const int eid = pk >> 5, r = pk & 31;
+    Acc9 acc;
+    elem_partial<S, Q, false>(f, mt, grad, eid, r, xs, nullptr, acc);
+    if (has) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) part[t0][i] = acc.v[i];
+    }
+}
+
 template <int S, int Q, bool LAT, class Grad>
 __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, const OwnersView& own,
                                             const MaterialView& mt, const Grad& grad, const float4* y, float damping,
@@ -461,11 +502,18 @@ __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, 
     const int first_slot = blk.x, n_slots = blk.y, first_own = blk.z;
     const int t0 = threadIdx.x;
     if constexpr (LAT) {
+        // the slot state rides along the element chain (a thread past the
+        // block's slots loads the last one and does not use it; every vertex
+        // block has a slot)
         SlotState st{};
-        if (t0 < n_slots) st = load_slot_state(ps, own, y, first_slot + t0);
-        fused_phase1<S, Q, dn_in_lds<S, Q, LAT>()>(ps, f, own, mt, grad, blk, part, dnw);
+        const int ts = first_slot + min(t0, n_slots - 1);
+        fused_phase1_lat<S, Q>(ps, f, own, mt, grad, blk, part, [&] { slot_state_level1(own, ts, st); },
+                               [&] { slot_state_level2(ps, y, st); });
         __syncthreads();
-        if (t0 < n_slots && st.free != 0) solve_slot_from_lds(ps, st, part, first_own, damping, dt);
+        if (t0 < n_slots) {
+            slot_state_settle(st);
+            if (st.free != 0) solve_slot_from_lds(ps, st, part, first_own, damping, dt);
+        }
     } else {
         fused_phase1<S, Q, dn_in_lds<S, Q, LAT>()>(ps, f, own, mt, grad, blk, part, dnw);
         __syncthreads();

@@ -65,17 +65,52 @@ constexpr Q2Tables make_tables() {
     }
     return t;
 }
-__constant__ Q2Tables kQ2 = make_tables();
+// The two tables packed 12 entries of 5 bits to a 64-bit word, read with
+// shifts: a per-lane index into a table in memory is a vector load, and a wait
+// for it would cover every load issued before it.
+constexpr unsigned long long pack_word(const int (&v)[32], int w) {
+    unsigned long long x = 0;
+    for (int i = 0; i < 12 && 12 * w + i < 27; ++i) x |= static_cast<unsigned long long>(v[12 * w + i]) << (5 * i);
+    return x;
+}
+constexpr Q2Tables kQ2Host = make_tables();
+constexpr unsigned long long kPlaceW0 = pack_word(kQ2Host.place, 0), kPlaceW1 = pack_word(kQ2Host.place, 1),
+                             kPlaceW2 = pack_word(kQ2Host.place, 2);
+constexpr unsigned long long kNodeW0 = pack_word(kQ2Host.node, 0), kNodeW1 = pack_word(kQ2Host.node, 1),
+                             kNodeW2 = pack_word(kQ2Host.node, 2);
+__device__ __forceinline__ int unpack27(unsigned long long w0, unsigned long long w1, unsigned long long w2, int i) {
+    const unsigned long long w = i < 12 ? w0 : (i < 24 ? w1 : w2);
+    const int sh = 5 * (i < 12 ? i : (i < 24 ? i - 12 : i - 24));
+    return static_cast<int>((w >> sh) & 31ull);
+}
+constexpr bool packed_tables_match() {
+    for (int i = 0; i < 27; ++i) {
+        const int w = i / 12, sh = 5 * (i % 12);
+        const unsigned long long pw = w == 0 ? kPlaceW0 : (w == 1 ? kPlaceW1 : kPlaceW2);
+        const unsigned long long nw = w == 0 ? kNodeW0 : (w == 1 ? kNodeW1 : kNodeW2);
+        if (static_cast<int>((pw >> sh) & 31ull) != kQ2Host.place[i]) return false;
+        if (static_cast<int>((nw >> sh) & 31ull) != kQ2Host.node[i]) return false;
+    }
+    return true;
+}
+static_assert(packed_tables_match(), "packed Q2 node tables");
 
 // The three 1D factors and their derivatives at one Gauss coordinate.
 struct Basis3 {
     float l0, l1, l2, d0, d1, d2;
 };
+// The factors are read as scalars and selected per lane: a select of the
+// argument words themselves would become a load with a per-lane address.
+__device__ __forceinline__ float uniform(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
 __device__ __forceinline__ Basis3 basis_at(const q2::Basis1D& B, int i) {
     const bool i0 = i == 0, i1 = i == 1;
-    return Basis3{i0 ? B.L[0][0] : (i1 ? B.L[0][1] : B.L[0][2]), i0 ? B.L[1][0] : (i1 ? B.L[1][1] : B.L[1][2]),
-                  i0 ? B.L[2][0] : (i1 ? B.L[2][1] : B.L[2][2]), i0 ? B.D[0][0] : (i1 ? B.D[0][1] : B.D[0][2]),
-                  i0 ? B.D[1][0] : (i1 ? B.D[1][1] : B.D[1][2]), i0 ? B.D[2][0] : (i1 ? B.D[2][1] : B.D[2][2])};
+    auto pick = [&](const float (&v)[3]) {
+        const float a = uniform(v[0]), b = uniform(v[1]), c = uniform(v[2]);
+        return i0 ? a : (i1 ? b : c);
+    };
+    return Basis3{pick(B.L[0]), pick(B.L[1]), pick(B.L[2]), pick(B.D[0]), pick(B.D[1]), pick(B.D[2])};
 }
 
 // One half's LDS.  Rows 27-31 are written by the idle lanes and never read.
@@ -135,8 +170,8 @@ __device__ __forceinline__ LanePlan lane_plan(const q2::Basis1D& B) {
     const int lc = p.l < 27 ? p.l : 26;
     const int p0 = lc / 9, p1 = (lc / 3) % 3, p2 = lc % 3;
     p.node = lc;
-    p.place = kQ2.place[lc];
-    p.q = kQ2.node[lc];
+    p.place = unpack27(kPlaceW0, kPlaceW1, kPlaceW2, lc);
+    p.q = unpack27(kNodeW0, kNodeW1, kNodeW2, lc);
     p.a9 = p0 * 9;
     p.x0 = p1 * 3 + p2;
     p.s1 = basis_at(B, p2);
@@ -216,16 +251,35 @@ __device__ __forceinline__ void q2_task_body(const FemView& f, const MaterialVie
 // displacement, the next task's node index and the one after's owners entry
 // are issued before task t's stages; indices past the block's last incidence
 // are clamped to it.  Partials go to part[task] (nine lanes, one float each).
+// after_owner / after_topo run right after the first owners-entry and node-id
+// loads: the latency form issues its slot state there.  A wave whose two
+// halves have no task returns at once (it holds no slot either).  Past that the
+// code is straight-line: a half without a task works on the block's last one
+// and stores nothing, so that no wait for a load has to cover the loads of
+// another path, which would make the compiler wait for every load in flight.
+template <class AfterOwner, class AfterTopo>
 __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, const MaterialView& mt,
                                            const LanePlan& P, const int* __restrict__ own, int first, int count,
-                                           int task0, TaskLDS& sh, float (*part)[9]) {
-    if (task0 >= count) return;   // uniform over the half
+                                           int task0, TaskLDS& sh, float (*part)[9], AfterOwner&& after_owner,
+                                           AfterTopo&& after_topo) {
+    if (__builtin_amdgcn_readfirstlane(task0 & ~1) >= count) return;
+    const bool idle = task0 >= count;   // uniform over the half
+    const int start = idle ? count - 1 : task0;
     const int last = first + count - 1;
-    int pk = gload(own, first + task0);
-    int pk_next = gload(own, min(first + task0 + kHalves, last));
+    int pk = gload(own, first + start);
+    int pk_next = gload(own, min(first + start + kHalves, last));
+    // the scheduler would hoist the hooks' loads (which do not depend on the
+    // chain) above the chain's own, and every wait of the chain would then
+    // cover them
+    __builtin_amdgcn_sched_barrier(0);
+    after_owner();
+    __builtin_amdgcn_sched_barrier(0);
     int vid = gload(f.topo, static_cast<size_t>(pk >> 5) * 27 + P.node);
+    __builtin_amdgcn_sched_barrier(0);
+    after_topo();
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll 1
-    for (int task = task0; task < count; task += kHalves) {
+    for (int task = start; task < count; task += kHalves) {
         const float4 u = gload(static_cast<const float4*>(ps.p), vid);
         const int eid = pk >> 5, r = pk & 31;
         pk = pk_next;
@@ -241,7 +295,7 @@ __device__ __forceinline__ void q2_task_run(const PSView& ps, const FemView& f, 
         s[4] = half_sum_pair(acc.v[8], 0.f);
         // lane k of row w writes value 2k + w
         const int k = P.l & 15, i = 2 * k + (P.l >> 4);
-        if (k < 5 && i < 9) {
+        if (!idle && k < 5 && i < 9) {
             float v = s[0];
 #pragma unroll
             for (int j = 1; j < 5; ++j) v = k == j ? s[j] : v;
@@ -270,17 +324,24 @@ __device__ __forceinline__ void q2_block(const PSView& ps, const FemView& fem, c
     __shared__ float part[LAT ? kQ2FusedIncPerBlockLat : kQ2FusedIncPerBlock][9];
     const int half = threadIdx.x >> 5;
     if constexpr (LAT) {
+        // the slot state rides along the element chain (a thread past the
+        // block's slots loads the last one and does not use it; every vertex
+        // block has a slot)
+        const int t = static_cast<int>(threadIdx.x);
         SlotState st{};
-        if (static_cast<int>(threadIdx.x) < blk.y) st = load_slot_state(ps, own, y, blk.x + threadIdx.x);
+        const int ts = blk.x + min(t, blk.y - 1);
         const LanePlan P = lane_plan(basis);
-        q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
+        q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part,
+                    [&] { slot_state_level1(own, ts, st); }, [&] { slot_state_level2(ps, y, st); });
         __syncthreads();
-        if (static_cast<int>(threadIdx.x) < blk.y && st.free != 0)
-            solve_slot_from_lds(ps, st, part, blk.z, damping, dt);
+        if (t < blk.y) {
+            slot_state_settle(st);
+            if (st.free != 0) solve_slot_from_lds(ps, st, part, blk.z, damping, dt);
+        }
     } else {
         const int wave_first = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) & ~63);
         const LanePlan P = lane_plan(basis);
-        q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part);
+        q2_task_run(ps, fem, mt, P, own.own, blk.z, blk.w, half, sh[half], part, [] {}, [] {});
         __syncthreads();
         q2_solve_slots(ps, own, y, damping, dt, blk, part, wave_first);
     }
