@@ -252,12 +252,11 @@ __device__ __forceinline__ void solve_slot_from_lds(const PSView& ps, const Slot
 
 // Phase 2 of the fused task kernels for one slot: add the slot's partials
 // (part[j], j = its owners index - first_own, in the owners order) and solve.
-__device__ __forceinline__ void solve_slot_direct(const PSView& ps, const OwnersView& own, const float4* y,
-                                                  float damping, float dt, int slot, const float (*part)[9],
-                                                  int first_own) {
-    const int vid = gload(own.slot_vid, slot);
+// The same with the slot's vertex id and (incidence count, first owners
+// index) already loaded.
+__device__ __forceinline__ void solve_slot_at(const PSView& ps, int vid, int2 nbo, const float4* y, float damping,
+                                              float dt, const float (*part)[9], int first_own) {
     if (gload(ps.mask, vid) == 0) return;
-    const int2 nbo = gload(own.slot_nbo, slot);
     Acc9 acc;
     acc_zero(acc);
     for (int j = nbo.y - first_own, e = j + nbo.x; j < e; ++j) {
@@ -270,6 +269,12 @@ __device__ __forceinline__ void solve_slot_direct(const PSView& ps, const Owners
     vbd_local_solve(acc, p, dunpack(gload(static_cast<const float4*>(ps.last_p), vid), un), dunpack(gload(y, vid), un),
                     gload(ps.m, vid), damping, dt, dx);
     gstore(ps.p, vid, dpack(dvec3{p.x + dx[0], p.y + dx[1], p.z + dx[2]}, ps.lo_inv));
+}
+
+__device__ __forceinline__ void solve_slot_direct(const PSView& ps, const OwnersView& own, const float4* y,
+                                                  float damping, float dt, int slot, const float (*part)[9],
+                                                  int first_own) {
+    solve_slot_at(ps, gload(own.slot_vid, slot), gload(own.slot_nbo, slot), y, damping, dt, part, first_own);
 }
 
 }  // namespace oe

@@ -272,11 +272,17 @@ __device__ __forceinline__ void pencil_block(const PSView& ps, const FemView& fe
     }
     if (lane < 27) pc.place[lane] = kPencil.place[lane];
     const int own_l = lane < blk.w ? gload(own.own, blk.z + lane) : 0;
+    // the slots' vertex ids and owners ranges, loaded now rather than after
+    // the rounds (one dependent load less at the block's end; a lane past the
+    // slots loads the last one, and every vertex block has a slot)
+    const int my_slot = blk.x + min(lane, blk.y - 1);
+    const int slot_vid = gload(own.slot_vid, my_slot);
+    const int2 slot_nbo = gload(own.slot_nbo, my_slot);
     int nid[3];
     pencil_nodes(fem, own_l, g, m, nid);
     for (int round = 0; round < kRounds && round * kGroups < blk.w; ++round)
         pencil_round(ps, fem, own_l, nid, mt, basis, pc, wq_i, blk.w, round, g, m, mine, part);
-    if (lane < blk.y) solve_slot_direct(ps, own, y, damping, dt, blk.x + lane, part, blk.z);
+    if (lane < blk.y) solve_slot_at(ps, slot_vid, slot_nbo, y, damping, dt, part, blk.z);
 }
 
 __global__ __launch_bounds__(64) void k_q2_pencil(Q2FusedArgs a) {
