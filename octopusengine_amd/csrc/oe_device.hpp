@@ -277,7 +277,7 @@ struct OwnersView {
 };
 
 // What the vertex solve of one slot reads.  The latency forms of the fused
-// task kernels load it before the element work of their block, so that the
+// task kernels load it during the element work of their block, so that the
 // loads overlap that work (a slot's vertex is written only by the block that
 // solves it).
 struct SlotState {

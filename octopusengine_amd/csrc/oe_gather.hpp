@@ -92,7 +92,7 @@ struct BatchGatherArgs {
 // Two forms share the kernels: the throughput form (each thread takes up to
 // kFusedRuns incidences in turn) and the latency form for launches too small
 // to fill the GPU (one incidence per thread, so a block's critical path holds
-// one element; the vertex state is loaded before the element work).  Each
+// one element; the vertex state is loaded along the element chain).  Each
 // form has its own vertex blocks.  One run per thread measured fastest for the
 // throughput form too (P2 batch: 11.25 ms per step against 11.50 ms for two,
 // with no loads run ahead: those held 12 VGPRs and cost a wave per SIMD), so

@@ -15,7 +15,7 @@ namespace oe {
 // index, incidence count}, colour-major) with at most kQ2FusedIncPerBlock
 // incidences and kQ2FusedMaxSlots slots.
 // The latency form (launches too small to fill the GPU) takes one task per
-// half-wave and loads the vertex state before the element work; it has its own
+// half-wave and loads the vertex state along the element chain; it has its own
 // vertex blocks of at most kQ2FusedIncPerBlockLat incidences.
 constexpr int kQ2FusedThreads = 256;
 // 4 tasks per half-wave.  8 are 2% faster on a large batch but double the
