@@ -515,11 +515,17 @@ __device__ __forceinline__ void fused_block(const PSView& ps, const FemView& f, 
             if (st.free != 0) solve_slot_from_lds(ps, st, part, first_own, damping, dt);
         }
     } else {
+        // the slots' vertex ids and owners ranges, loaded now rather than after
+        // the barrier (one dependent load less in phase 2; a thread past the
+        // slots loads the last one)
+        const int ts = first_slot + min(t0, n_slots - 1);
+        const int slot_vid = gload(own.slot_vid, ts);
+        const int2 slot_nbo = gload(own.slot_nbo, ts);
         fused_phase1<S, Q, dn_in_lds<S, Q, LAT>()>(ps, f, own, mt, grad, blk, part, dnw);
         __syncthreads();
         // phase 2: one thread per slot, partials in the owners order, local solve
         if (t0 >= n_slots) return;
-        solve_slot_direct(ps, own, y, damping, dt, first_slot + t0, part, first_own);
+        solve_slot_at(ps, slot_vid, slot_nbo, y, damping, dt, part, first_own);
     }
 }
 
