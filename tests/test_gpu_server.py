@@ -102,6 +102,26 @@ def test_server_tile_sets_the_served_layout(engine):
         assert d_alone <= 1e-6
 
 
+def test_server_tile_refused_while_requests_are_in_flight(engine):
+    """The layout changes only while nothing is pending or in flight: a step of
+    5 substeps of this scene (about 1,450 dependent launches) takes
+    milliseconds, and the call right after the submission lands inside it."""
+    sc = _scene(engine, (oe.HEXA27, (16, 4, 4), 1e6, 0.45))
+    srv = oe.Server(engine, max_batch=2, streams=1)
+    slot = srv.add_scene(sc)
+    srv.submit(slot, 0.005, 5)
+    with pytest.raises(oe.OEError):
+        srv.set_tile_ex(oe.HEXA27, 64, 1, oe.PASS_TASKS, 28, 0)
+    srv.drain()
+    srv.set_tile_ex(oe.HEXA27, 64, 1, oe.PASS_TASKS, 28, 0)
+    assert srv.get_tile_ex(oe.HEXA27)["inc_per_block"] == 28
+    srv.submit(slot, 0.005, 1)
+    srv.drain()
+    assert np.isfinite(sc.positions()).all()
+    srv.close()
+    sc.close()
+
+
 def test_server_rejects_bad_requests(engine):
     s = _scene(engine, SPECS[0])
     srv = oe.Server(engine, max_batch=2, streams=1)
