@@ -163,6 +163,7 @@ def batched_leg(args, eng) -> dict:
     nv = int(sum(s.n_vertices for s in scenes))
     n_p2 = sum(1 for s in scenes if s.element == oe.TETRA10)
     b = oe.Batch(eng, scenes)
+    b.set_streams(args.batch_streams)
     info = b.info()
     vu_step = info["vertex_updates"]
     for _ in range(args.warmup):
@@ -179,7 +180,9 @@ def batched_leg(args, eng) -> dict:
     # timed region (value): eager launches, as Batch.step issues them
     wall = timed(lambda: b.step(DT, 1), args.steps, eng.synchronize)
     # roofline: as many steps again with HIP events around every colour-pass
-    # launch, on the stream the kernels run on (kernel durations only)
+    # launch, on the stream the kernels run on (kernel durations only; with
+    # the events on, every pass runs on one stream, so each duration is the
+    # kernel's own and not shared with a pass beside it)
     b.set_timing(True)
     b.get_timing()
     q2_pencil = b.get_tile_ex(oe.HEXA27)["inc_per_block"] in Q2_PENCIL_INC
@@ -562,6 +565,8 @@ def main():
                     help="throughput-form blocks below which a served Q2 colour runs the latency form (-1: default)")
     ap.add_argument("--eager-only", action="store_true",
                     help="batched leg: only the eager timed region (no graph, latency or plain loop)")
+    ap.add_argument("--batch-streams", type=int, default=2, choices=(1, 2),
+                    help="streams of a batch step: 2 runs the Q2 colour passes beside the P2 ones")
     ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
     ap.add_argument("--single-only", action="store_true",
                     help="run only the single-scene leg (P2 64x16x16) and print its figures (profiling)")
@@ -607,8 +612,10 @@ def main():
                                f"E~logU[1e5,1e7], nu~U[0.3,0.49], SNH, dt=0.005, 10 iterations, seed {args.seed}",
                    "vertices": bat["vertices"], "vertex_updates_per_step": bat["vertex_updates_per_step"],
                    "gather_launches_per_step": bat["gather_launches_per_step"],
-                   "launch": "eager (value); roofline from a second run of as many steps with HIP events "
-                             "around every colour-pass launch, for the kernel with the largest share of a step "
+                   "batch_streams": args.batch_streams,
+                   "launch": "eager (value; with 2 batch streams the Q2 colour passes run beside the P2 ones); "
+                             "roofline from a second run of as many steps with HIP events around every colour-pass "
+                             "launch, all on one stream, for the kernel with the largest share of a step "
                              "(every colour-pass kernel in roofline_by_kernel)"},
         "roofline": roof,
         "roofline_by_kernel": roof_by_kernel,

@@ -196,6 +196,12 @@ oe_status oe_batch_create(oe_engine* e, oe_scene* const* scenes, int32_t n, oe_b
 oe_status oe_batch_destroy(oe_batch* b);
 oe_status oe_batch_step(oe_batch* b, float dt, int32_t substeps);
 oe_status oe_batch_set_graph(oe_batch* b, int32_t enable);
+/* Streams of a batch step: 2 (default) runs the 27-node hexahedra's colour
+ * passes on a second stream beside the other element types' (the two chains
+ * are independent within an iteration and fill each other's idle issue
+ * slots), 1 runs every pass on the engine's stream.  The results are the same
+ * either way.  No counterpart in the reference. */
+oe_status oe_batch_set_streams(oe_batch* b, int32_t n);
 oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t mode);
 /* Full layout per element type (latency_blocks has no effect: batches run the
  * throughput form); get returns the layout in use. */

@@ -504,6 +504,10 @@ class Batch:
     def set_graph(self, enable: bool):
         L.check(L.lib.oe_batch_set_graph(self._h, int(enable)))
 
+    def set_streams(self, n: int):
+        """1: every colour pass on the engine's stream; 2 (default): Q2 passes on a second stream."""
+        L.check(L.lib.oe_batch_set_streams(self._h, n))
+
     def set_tile(self, element: int, lanes: int, qpl: int, mode: int = PASS_GATHER):
         L.check(L.lib.oe_batch_set_tile(self._h, element, lanes, qpl, int(mode)))
 

@@ -117,6 +117,7 @@ _SIGS = {
     "oe_batch_destroy": (C.c_int32, [P]),
     "oe_batch_step": (C.c_int32, [P, C.c_float, C.c_int32]),
     "oe_batch_set_graph": (C.c_int32, [P, C.c_int32]),
+    "oe_batch_set_streams": (C.c_int32, [P, C.c_int32]),
     "oe_batch_set_tile": (C.c_int32, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "oe_batch_set_timing": (C.c_int32, [P, C.c_int32]),
     "oe_batch_get_timing": (C.c_int32, [P, f64p, i64p]),

@@ -266,9 +266,26 @@ GatherTimer::~GatherTimer() {
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
 
+void SideStream::create() {
+    if (stream) return;
+    OE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    OE_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    OE_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+}
+
+SideStream::~SideStream() {
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
 void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const SceneDev* d_scenes, float dt, int substeps,
-                     hipStream_t stream, GatherTimer* timer) {
+                     hipStream_t stream, GatherTimer* timer, SideStream* side) {
     if (sch.vsegs.empty()) return;
+    auto is_q2 = [](const Schedule::GLaunch& gl) { return gl.S == 27 && gl.Q == 27; };
+    bool any_q2 = false, any_other = false;
+    for (const Schedule::GLaunch& gl : sch.glaunches) (is_q2(gl) ? any_q2 : any_other) = true;
+    const bool split = side && side->stream && !timer && any_q2 && any_other;
     const float h = dt / static_cast<float>(substeps);
     BatchVArgs v{d_scenes, dev.vsegs.get(), static_cast<int>(sch.vsegs.size()), h, 0};
     BatchGatherArgs g{d_scenes, dev.segs.get(), 0, 0, 0, h};
@@ -285,29 +302,37 @@ void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const Scene
                 if (!launch_batch_ju_refresh(jl.S, jl.Q, r, jl.nblocks, stream))
                     throw std::invalid_argument("unsupported element in batch");
             }
+            if (split) {
+                OE_HIP(hipEventRecord(side->fork, stream));
+                OE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+            }
             for (const Schedule::GLaunch& gl : sch.glaunches) {
+                const hipStream_t ls = split && is_q2(gl) ? side->stream : stream;
                 g.seg_first = gl.seg_first;
                 g.seg_count = gl.seg_count;
-                if (timer) timer->before(stream, gl.element);
-                if (gl.mode == PASS_TASKS && gl.S == 27 && gl.Q == 27) {
+                if (timer) timer->before(ls, gl.element);
+                if (gl.mode == PASS_TASKS && is_q2(gl)) {
                     tk.seg_first = gl.seg_first;
                     tk.seg_count = gl.seg_count;
                     tk.latency = gl.latency ? 1 : 0;
                     if (!gl.latency && q2_pencil(gl.inc))
-                        launch_batch_q2_pencil(tk, gl.nblocks, stream);
+                        launch_batch_q2_pencil(tk, gl.nblocks, ls);
                     else
-                        launch_batch_q2_fused(tk, gl.nblocks, stream);
+                        launch_batch_q2_fused(tk, gl.nblocks, ls);
                 } else if (gl.mode == PASS_TASKS) {
                     ft.seg_first = gl.seg_first;
                     ft.seg_count = gl.seg_count;
                     ft.latency = gl.latency ? 1 : 0;
-                    if (!launch_batch_fused_tasks(gl.S, gl.Q, ft, gl.nblocks, stream))
+                    if (!launch_batch_fused_tasks(gl.S, gl.Q, ft, gl.nblocks, ls))
                         throw std::invalid_argument("unsupported element in batch");
-                } else if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.mode == PASS_CACHE, g, gl.nblocks,
-                                                stream)) {
+                } else if (!launch_batch_gather(gl.S, gl.Q, gl.L, gl.QL, gl.mode == PASS_CACHE, g, gl.nblocks, ls)) {
                     throw std::invalid_argument("unsupported gather tile in batch");
                 }
-                if (timer) timer->after(stream);
+                if (timer) timer->after(ls);
+            }
+            if (split) {
+                OE_HIP(hipEventRecord(side->join, side->stream));
+                OE_HIP(hipStreamWaitEvent(stream, side->join, 0));
             }
             v.iteration = it;
             launch_batch_fix_box(v, sch.vblocks, stream);
@@ -331,6 +356,7 @@ Batch::Batch(Engine* e, std::vector<Scene*> sc) : eng(e), scenes(std::move(sc)),
     flags.reserve(scenes.size());
     for (Scene* s : scenes) flags.emplace_back(&eng->pool, fixed_flags(*s));
     d_scenes.allocate(&eng->pool, scenes.size());
+    side.create();
     rebuild();
     for (Scene* s : scenes) s->batches.push_back(this);
 }
@@ -373,6 +399,11 @@ void Batch::set_graph(bool on) {
     use_graph = on;
 }
 
+void Batch::set_streams(int n) {
+    if (n != streams) drop_graph();   // a captured graph holds the old launch structure
+    streams = n;
+}
+
 void Batch::drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -384,13 +415,15 @@ void Batch::step(float dt, int substeps) {
     if (broken) throw std::logic_error("a scene of this batch was destroyed");
     if (!(dt > 0.f) || substeps <= 0) throw std::invalid_argument("step needs dt > 0 and substeps > 0");
     if (!use_graph || timer.enabled) {
-        launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, timer.enabled ? &timer : nullptr);
+        launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, timer.enabled ? &timer : nullptr,
+                        streams == 2 ? &side : nullptr);
     } else {
         if (!graph_exec || graph_dt != dt || graph_sub != substeps) {
             drop_graph();
             OE_HIP(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
             try {
-                launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, nullptr);
+                launch_schedule(sched, dsched, d_scenes.get(), dt, substeps, eng->stream, nullptr,
+                                streams == 2 ? &side : nullptr);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(eng->stream, &g);

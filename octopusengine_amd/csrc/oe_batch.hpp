@@ -99,9 +99,23 @@ struct GatherTimer {
     ~GatherTimer();
 };
 
+// A second stream for the Q2 colour passes of a schedule that also holds other
+// element types: the two chains of colour passes are independent within an
+// iteration, so they run side by side and fill each other's idle issue slots
+// (the Q2 pass is VALU-bound, the P2 pass waits on loads).  The fork and join
+// events order them against the per-iteration passes on the main stream.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    void create();
+    ~SideStream();
+};
+
 // Enqueue `substeps` substeps of dt/substeps for every scene of the schedule.
+// With `side` (and no timer) a schedule with Q2 and other element types runs
+// its Q2 colour passes on side->stream.
 void launch_schedule(const Schedule& sch, const DeviceSchedule& dev, const SceneDev* d_scenes, float dt, int substeps,
-                     hipStream_t stream, GatherTimer* timer);
+                     hipStream_t stream, GatherTimer* timer, SideStream* side = nullptr);
 
 // The default tile of every element type: a scene's (default_tile) for a
 // server's ticks, and for a batch the same but 28-incidence Q2 pencil blocks.
@@ -116,6 +130,7 @@ struct Batch {
     ~Batch();
     void step(float dt, int substeps);
     void set_graph(bool on);
+    void set_streams(int n);   // 1 or 2 (the Q2 passes on a second stream)
     void set_tile(Element e, GatherTile t);
     void rebuild();
     void scene_destroyed(Scene* s);
@@ -128,10 +143,12 @@ struct Batch {
     Schedule sched;
     DeviceSchedule dsched;
     GatherTimer timer;
+    SideStream side;
 
 private:
     bool broken = false;   // a scene of the batch was destroyed
     bool use_graph = false;
+    int streams = 2;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     float graph_dt = 0.f;

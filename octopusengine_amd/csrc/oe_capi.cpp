@@ -557,6 +557,14 @@ oe_status oe_batch_set_graph(oe_batch* b, int32_t enable) {
     });
 }
 
+oe_status oe_batch_set_streams(oe_batch* b, int32_t n) {
+    return guard([&] {
+        OE_REQUIRE(b, "null batch");
+        OE_REQUIRE(n == 1 || n == 2, "a batch steps on 1 or 2 streams");
+        b->batch->set_streams(n);
+    });
+}
+
 oe_status oe_batch_set_tile(oe_batch* b, int32_t element, int32_t lanes, int32_t qpl, int32_t mode) {
     return guard([&] {
         OE_REQUIRE(b && oe::element_supported(element), "bad argument");

@@ -84,6 +84,28 @@ def test_batch_graph_replay_matches_eager(engine):
             assert max_err(a.positions(), c.positions()) == 0.0, f"replay {i}"
 
 
+def test_batch_streams_give_the_same_steps(engine):
+    """One stream, or the Q2 colour passes on a second stream beside the
+    others (eager and graph replay): the same positions, bit for bit."""
+    s1 = [_make(engine, s)[0] for s in MIX]
+    s2 = [_make(engine, s)[0] for s in MIX]
+    s3 = [_make(engine, s)[0] for s in MIX]
+    b1, b2, b3 = oe.Batch(engine, s1), oe.Batch(engine, s2), oe.Batch(engine, s3)
+    b1.set_streams(1)
+    b3.set_graph(True)   # two streams (the default), captured
+    with pytest.raises(oe.OEError):
+        b2.set_streams(3)
+    for i in range(3):
+        for b in (b1, b2, b3):
+            b.step(0.005, 1)
+        engine.synchronize()
+        for a, c, d in zip(s1, s2, s3):
+            assert max_err(a.positions(), c.positions()) == 0.0, f"step {i}: two streams, eager"
+            assert max_err(a.positions(), d.positions()) == 0.0, f"step {i}: two streams, graph"
+    for b in (b1, b2, b3):
+        b.close()
+
+
 def test_batch_tiles_and_timing(engine):
     scenes = [_make(engine, s)[0] for s in MIX[:2]]
     ref = [_make(engine, s)[0] for s in MIX[:2]]
