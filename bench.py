@@ -255,6 +255,42 @@ def batched_leg(args, eng) -> dict:
     }
 
 
+def config3_leg(eng) -> dict:
+    """The high-order explicit and XPBD workload on the GPU: Q2 and P3 beams of
+    32 x 8 x 8 cells, Neo-Hookean E = 1e6 Pa, nu = 0.45, +-1e-3 m initial noise,
+    x = 0 face fixed; XPBD frames of dt = 0.01 s with 50 substeps (V2 kernels)
+    and explicit substeps of 1e-5 s (timed as 10 calls of 200 substeps)."""
+    import octopusengine_amd as oe
+
+    out = {}
+    for el, name in ((oe.HEXA27, "q2"), (oe.TETRA20, "p3")):
+        xyz, topo = oe.beam_mesh(el, (32, 8, 8), (1.0, 0.25, 0.25))
+        rng = np.random.default_rng(11)
+        p0 = (xyz + rng.uniform(-1e-3, 1e-3, xyz.shape)).astype(np.float32)
+        for solver, kind in ((oe.SOLVER_XPBD, "xpbd"), (oe.SOLVER_EXPLICIT, "explicit")):
+            sc = oe.Scene(eng, el, xyz, topo, material=oe.NEOHOOKE, young=1e6, poisson=0.45, density=1000.0,
+                          distribution=oe.MASS_FEMSHAPE, solver=solver, xpbd_version=2)
+            sc.fix_plane((1e-6, 0.0, 0.0), (-1.0, 0.0, 0.0))
+            sc.box_limit((-10.0, -5.0, -10.0), (10.0, 5.0, 10.0))
+            sc.set_state(p=p0, last_p=p0)
+            if kind == "xpbd":
+                dt, sub, calls = 0.01, 50, 5
+            else:
+                dt, sub, calls = 200 * 1e-5, 200, 10
+            sc.step(dt, sub)
+            wall = timed(lambda: sc.step(dt, sub), calls, eng.synchronize)
+            n = sc.n_vertices
+            rec = {"vertices": n, "substeps_per_call": sub, "calls": calls,
+                   "us_per_substep": wall / (calls * sub) * 1e6,
+                   "vertex_substeps_per_s": n * sub * calls / wall,
+                   "positions_finite": bool(np.isfinite(sc.positions()).all())}
+            if kind == "xpbd":
+                rec["ms_per_frame"] = wall / calls * 1e3
+            out[f"{kind}_{name}_32x8x8"] = rec
+            sc.close()
+    return out
+
+
 def plain_single_leg(args, eng) -> dict:
     import octopusengine_amd as oe
 
@@ -568,6 +604,8 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=2, choices=(1, 2),
                     help="streams of a batch step: 2 runs the Q2 colour passes beside the P2 ones")
     ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the explicit / XPBD leg (Q2 and P3 32x8x8) and print its figures (profiling)")
     ap.add_argument("--single-only", action="store_true",
                     help="run only the single-scene leg (P2 64x16x16) and print its figures (profiling)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -582,6 +620,10 @@ def main():
     eng = oe.Engine(0)
     if args.single_only:
         print(json.dumps(_finite_or_null({"plain_single_scene_p2_64x16x16": plain_single_leg(args, eng)})), flush=True)
+        eng.close()
+        return
+    if args.config3_only:
+        print(json.dumps(_finite_or_null({"config3": config3_leg(eng)})), flush=True)
         eng.close()
         return
     if args.serve_only:
@@ -647,6 +689,9 @@ def main():
         eng = oe.Engine(0)
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
         progress(f"single-scene leg: {out['plain_single_scene_p2_64x16x16']['ms_per_step']:.3f} ms per step")
+        out["config3"] = config3_leg(eng)
+        progress("config-3 leg: " + ", ".join(f"{k} {v['us_per_substep']:.1f} us/substep"
+                                              for k, v in out["config3"].items()))
         eng.close()
     if args.copy_bytes > 0:
         out["copy_calibration"] = copy_calibration(args.copy_bytes)

@@ -113,7 +113,10 @@ oe_status oe_scene_set_graph(oe_scene* s, int32_t enable);
  * 1 (gather with the displacement gradients of every (element, quadrature
  * point) kept in device memory during a sweep; qpl must be 1) or 2 (element
  * tasks - one per incidence of the colour, sum-factorised gradients for
- * 27-node hexahedra - then vertex solves).  Modes 1 and 2 are VBD options.  No counterpart in the reference,
+ * 27-node hexahedra - then vertex solves).  Mode 1 is a VBD option.  On an
+ * explicit scene mode 2 (the default) selects the element-centric force pass
+ * (one element per 32 lanes, then a per-vertex sum of the element partials)
+ * and mode 0 the per-vertex gather with this tile.  No counterpart in the reference,
  * whose kernels fix the thread layout per version
  * (src/GPU/VBD/GPU_VBD_FEM.cu:712-756). */
 oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mode);

@@ -294,8 +294,9 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
         oe::Scene& sc = *s->scene;
         require_unshared(sc, "set_tile");
         check_tile(sc.mesh.element, lanes, qpl, mode);
-        OE_REQUIRE(mode == oe::PASS_GATHER || sc.vbd_fem, "cached gradients and element tasks are VBD options");
-        OE_REQUIRE(mode != oe::PASS_TASKS || sc.mesh.element != oe::Hexa27 || sc.vbd_fem->has_q2_basis,
+        OE_REQUIRE(mode == oe::PASS_GATHER || sc.vbd_fem || (mode == oe::PASS_TASKS && sc.explicit_fem),
+                   "cached gradients are a VBD option; element tasks a VBD or explicit one");
+        OE_REQUIRE(mode != oe::PASS_TASKS || !sc.vbd_fem || sc.mesh.element != oe::Hexa27 || sc.vbd_fem->has_q2_basis,
                    "no tensor factors for this 27-node hexahedron");
         sc.set_graph(false);
         if (sc.vbd_fem) {
@@ -309,7 +310,7 @@ oe_status oe_scene_set_tile(oe_scene* s, int32_t lanes, int32_t qpl, int32_t mod
                 throw;
             }
         }
-        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, oe::PASS_GATHER};
+        if (sc.explicit_fem) sc.explicit_fem->tile = oe::GatherTile{lanes, qpl, mode};
     });
 }
 
@@ -341,8 +342,10 @@ oe_status oe_scene_set_tile_ex(oe_scene* s, const oe_tile* t) {
         oe::Scene& sc = *s->scene;
         require_unshared(sc, "set_tile");
         check_tile_ex(sc.mesh.element, t);
-        OE_REQUIRE(t->mode == oe::PASS_GATHER || sc.vbd_fem, "cached gradients and element tasks are VBD options");
-        OE_REQUIRE(t->mode != oe::PASS_TASKS || sc.mesh.element != oe::Hexa27 || sc.vbd_fem->has_q2_basis,
+        OE_REQUIRE(t->mode == oe::PASS_GATHER || sc.vbd_fem || (t->mode == oe::PASS_TASKS && sc.explicit_fem),
+                   "cached gradients are a VBD option; element tasks a VBD or explicit one");
+        OE_REQUIRE(t->mode != oe::PASS_TASKS || !sc.vbd_fem || sc.mesh.element != oe::Hexa27 ||
+                       sc.vbd_fem->has_q2_basis,
                    "no tensor factors for this 27-node hexahedron");
         oe::apply_scene_tile(sc, tile_from(t));
     });
@@ -412,6 +415,7 @@ oe_status oe_scene_get_tile(oe_scene* s, int32_t* lanes, int32_t* qpl, int32_t* 
             *mode = oe::PASS_GATHER;
             if (s->scene->vbd_fem && s->scene->vbd_fem->cached()) *mode = oe::PASS_CACHE;
             if (s->scene->vbd_fem && s->scene->vbd_fem->tasks()) *mode = oe::PASS_TASKS;
+            if (s->scene->explicit_fem) *mode = t.mode;
         }
     });
 }

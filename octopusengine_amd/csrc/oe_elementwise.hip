@@ -228,6 +228,33 @@ __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* sl
     ps.p[vid] = p_new[vid];
 }
 
+// Internal force of each vertex from the element partials of k_explicit_elem
+// (9 floats per (element, local node)): the force and Hessian blocks of the
+// vertex's owners entries are added in owners order, then the damping term
+// f = sum(-V P g) - k H v of the reference's force pass
+// (src/GPU/Explicit/GPU_FEM_Explicit.cu:45-62) overwrites ps.f.
+__global__ __launch_bounds__(kBlock) void k_explicit_vertex(PSView ps, OwnersView own, const float* partial, int S,
+                                                            int count, float damping) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const int vid = gload(own.slot_vid, slot);
+    const int2 nbo = gload(own.slot_nbo, slot);
+    float a[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < nbo.x; ++j) {
+        const int pk = gload(own.own, nbo.y + j);
+        const size_t at = (static_cast<size_t>(pk >> 5) * S + (pk & 31)) * 9;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[i] += gload(partial, at + i);
+    }
+    float4 f = make_float4(a[0], a[1], a[2], 0.f);
+    if (damping != 0.f) {
+        const float4 v = ps.v[vid];
+        f.x -= damping * (a[3] * v.x + a[4] * v.y + a[5] * v.z);
+        f.y -= damping * (a[4] * v.x + a[6] * v.y + a[7] * v.z);
+        f.z -= damping * (a[5] * v.x + a[7] * v.y + a[8] * v.z);
+    }
+    ps.f[vid] = f;
+}
 
 // ------------------------------------------------------------ batched forms
 // One launch covers every vertex of every scene of a batch; each block belongs
@@ -342,6 +369,13 @@ void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipS
     if (ps.n <= 0) return;
     hipLaunchKernelGGL(k_box, dim3(grid_for(ps.n)), dim3(kBlock), 0, s, ps, make_float3(pmin[0], pmin[1], pmin[2]),
                        make_float3(pmax[0], pmax[1], pmax[2]));
+}
+
+void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int S, int count,
+                            float damping, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_explicit_vertex, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ps, own, partial,
+                       S, count, damping);
 }
 
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,

@@ -611,6 +611,110 @@ bool launch_L(int L, const GatherArgs& a, hipStream_t s) {
     }
 }
 
+// Element-centric internal forces of the explicit integrator.  The reference
+// computes them per vertex (kernel_fem_eval_force,
+// src/GPU/Explicit/GPU_FEM_Explicit.cu:5-64), so a vertex rebuilds F at every point
+// of each incident element: S times over per (element, point).  Here each
+// 32-lane half of a wave takes one element:
+//   nodes:  lane k < S puts node k's displacement relative to node 0 in LDS;
+//   points: lane q < Q forms Ju = sum_k x_k (x) dN_k(q), F, cof F and
+//           s = lambda (J - alpha) once and puts them in LDS, point-major per
+//           component (lanes write consecutive words);
+//   nodes:  lane r < S adds the force -V (mu F g + s C g) and Hessian block of
+//           node r over the Q points (g = JX_inv^T dN_r(q); every lane of a
+//           half reads the same point words, a broadcast) and stores the 9
+//           floats at partial[(e S + r) 9].
+// k_explicit_vertex then adds each vertex's partials in owners order.
+constexpr int kExplicitHalves = 4;   // elements per block (128 threads)
+constexpr int kExplicitPt = 29;      // F 9, cof F 9, JX_inv 9, s, V
+
+template <int S, int Q>
+__global__ __launch_bounds__(kExplicitHalves * 32) void k_explicit_elem(ExplicitElemArgs a) {
+    static_assert(S <= 32 && Q <= 32, "one element per 32 lanes");
+    __shared__ float dn_s[Q * S * 3];
+    __shared__ float xs_s[kExplicitHalves][S * 3];
+    __shared__ float pt_s[kExplicitHalves][kExplicitPt][Q];
+    const int tid = threadIdx.x;
+    const int h = tid >> 5, lane = tid & 31;
+    const int E = a.fem.E;
+    const int e = blockIdx.x * kExplicitHalves + h;
+    const int ec = e < E ? e : E - 1;
+    for (int i = tid; i < Q * S * 3; i += kExplicitHalves * 32) dn_s[i] = gload(a.fem.dN, i);
+    {
+        const int k = lane < S ? lane : S - 1;
+        const int node = gload(a.fem.topo, static_cast<size_t>(ec) * S + k);
+        const float4 u = gload(static_cast<const float4*>(a.ps.p), node);
+        float4 ref;
+        ref.x = __shfl(u.x, 0, 32);
+        ref.y = __shfl(u.y, 0, 32);
+        ref.z = __shfl(u.z, 0, 32);
+        ref.w = __shfl(u.w, 0, 32);
+        float x[3];
+        rel_node(u, ref, a.ps.lo_unit, x);
+        if (lane < S) {
+            xs_s[h][3 * lane] = x[0];
+            xs_s[h][3 * lane + 1] = x[1];
+            xs_s[h][3 * lane + 2] = x[2];
+        }
+    }
+    const int q = lane < Q ? lane : Q - 1;
+    float JXi[9], V;
+    load_element_record(a.fem, ec, q, JXi, V);
+    __syncthreads();
+    {
+        float Ju[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ju[i] = 0.f;
+        const float* __restrict__ dn = dn_s + q * S * 3;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const float d0 = dn[3 * k], d1 = dn[3 * k + 1], d2 = dn[3 * k + 2];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const float xk = xs_s[h][3 * k + r];
+                Ju[0 + r] = fmaf(xk, d0, Ju[0 + r]);
+                Ju[3 + r] = fmaf(xk, d1, Ju[3 + r]);
+                Ju[6 + r] = fmaf(xk, d2, Ju[6 + r]);
+            }
+        }
+        float F[9], C[9];
+        deformation_gradient(Ju, JXi, F);
+        cofactor(F, C);
+        const float J = det_from_cof(F, C);
+        if (lane < Q) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                pt_s[h][i][lane] = F[i];
+                pt_s[h][9 + i][lane] = C[i];
+                pt_s[h][18 + i][lane] = JXi[i];
+            }
+            pt_s[h][27][lane] = a.mt.lambda * (J - a.mt.alpha);
+            pt_s[h][28][lane] = V;
+        }
+    }
+    __syncthreads();
+    if (e >= E || lane >= S) return;
+    Acc9 acc;
+    acc_zero(acc);
+    for (int p = 0; p < Q; ++p) {
+        float F[9], C[9], Xi[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            F[i] = pt_s[h][i][p];
+            C[i] = pt_s[h][9 + i][p];
+            Xi[i] = pt_s[h][18 + i][p];
+        }
+        const float* __restrict__ dn = dn_s + (p * S + lane) * 3;
+        float g[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[c] = Xi[c * 3 + 0] * dn[0] + Xi[c * 3 + 1] * dn[1] + Xi[c * 3 + 2] * dn[2];
+        nh_node(F, C, pt_s[h][27][p], pt_s[h][28][p], g, a.mt, acc);
+    }
+    float* out = a.partial + (static_cast<size_t>(e) * S + lane) * 9;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = acc.v[i];
+}
+
 }  // namespace
 
 #if defined(OE_GATHER_S) && defined(OE_GATHER_Q)
@@ -701,6 +805,12 @@ void OE_FN(launch_batch_fused_tasks_)(const BatchFusedArgs& b, int nblocks, hipS
 }
 
 #endif
+
+void OE_FN(launch_explicit_elem_)(const ExplicitElemArgs& a, hipStream_t s) {
+    if (a.fem.E <= 0) return;
+    hipLaunchKernelGGL((k_explicit_elem<OE_GATHER_S, OE_GATHER_Q>),
+                       dim3((a.fem.E + kExplicitHalves - 1) / kExplicitHalves), dim3(kExplicitHalves * 32), 0, s, a);
+}
 
 void OE_FN(launch_batch_ju_refresh_)(const BatchRefreshArgs& b, int nblocks, hipStream_t s) {
     constexpr int S = OE_GATHER_S, Q = OE_GATHER_Q;

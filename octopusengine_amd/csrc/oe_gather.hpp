@@ -143,6 +143,16 @@ struct BatchRefreshArgs {
     int iteration;
 };
 
+// Element-centric internal forces of the explicit integrator: per element and
+// local node r, the 9 floats (force, then the Hessian block xx xy xz yy yz zz)
+// of node r over the element's Q points at partial[(e S + r) 9].
+struct ExplicitElemArgs {
+    PSView ps;
+    FemView fem;
+    MaterialView mt;
+    float* partial;   // E * S * 9
+};
+
 #define OE_GATHER_DECL(S, Q)                                                                                  \
     bool launch_gather_##S##_##Q(int L, int QL, const GatherArgs& a, hipStream_t s);                          \
     bool launch_batch_gather_##S##_##Q(int L, int QL, bool cached, const BatchGatherArgs& b, int nblocks,     \
@@ -151,7 +161,8 @@ struct BatchRefreshArgs {
     void launch_batch_ju_refresh_##S##_##Q(const BatchRefreshArgs& b, int nblocks, hipStream_t s);          \
     int refresh_elements_per_block_##S##_##Q();                                                               \
     void launch_fused_tasks_##S##_##Q(const FusedTaskArgs& a, hipStream_t s);                                 \
-    void launch_batch_fused_tasks_##S##_##Q(const BatchFusedArgs& b, int nblocks, hipStream_t s);
+    void launch_batch_fused_tasks_##S##_##Q(const BatchFusedArgs& b, int nblocks, hipStream_t s);            \
+    void launch_explicit_elem_##S##_##Q(const ExplicitElemArgs& a, hipStream_t s);
 OE_GATHER_DECL(4, 1)
 OE_GATHER_DECL(8, 8)
 OE_GATHER_DECL(10, 4)
@@ -212,6 +223,16 @@ inline bool launch_batch_ju_refresh(int S, int Q, const BatchRefreshArgs& b, int
     else if (S == 10 && Q == 4) launch_batch_ju_refresh_10_4(b, nblocks, s);
     else if (S == 20 && Q == 11) launch_batch_ju_refresh_20_11(b, nblocks, s);
     else if (S == 27 && Q == 27) launch_batch_ju_refresh_27_27(b, nblocks, s);
+    else return false;
+    return true;
+}
+
+inline bool launch_explicit_elem(int S, int Q, const ExplicitElemArgs& a, hipStream_t s) {
+    if (S == 4 && Q == 1) launch_explicit_elem_4_1(a, s);
+    else if (S == 8 && Q == 8) launch_explicit_elem_8_8(a, s);
+    else if (S == 10 && Q == 4) launch_explicit_elem_10_4(a, s);
+    else if (S == 20 && Q == 11) launch_explicit_elem_20_11(a, s);
+    else if (S == 27 && Q == 27) launch_explicit_elem_27_27(a, s);
     else return false;
     return true;
 }

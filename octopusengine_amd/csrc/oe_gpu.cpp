@@ -580,13 +580,14 @@ void GPU_PBD_FEM::step(GPU_ParticleSystem* ps, float dt) {
 // ------------------------------------------------------- GPU_FEM_Explicit
 GPU_FEM_Explicit::GPU_FEM_Explicit(Engine* e, Element el, const HostMesh& mesh, float young, float poisson,
                                    Material material, float damp)
-    : GPU_FEM(e, el, mesh, young, poisson, material), _damping(damp), tile(default_tile(el)),
+    : GPU_FEM(e, el, mesh, young, poisson, material), _damping(damp), tile{64, 1, PASS_TASKS},
       n_vertices(mesh.nb_vertices()) {
     Coloration one;
     one.nb_color = 1;
     one.colors.assign(n_vertices, 0);
     const OwnersCSR o = build_owners(mesh, one);   // every vertex, ascending id
     upload_owners(eng, o, d_slot_vid, d_slot_nbo, d_own);
+    d_partial = DBuf<float>(&eng->pool, static_cast<size_t>(mesh.nb_elements()) * S * 9);
 }
 
 void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
@@ -601,6 +602,12 @@ void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
     a.first_slot = 0;
     a.count = n_vertices;
     a.mode = GATHER_EXPLICIT_FORCE;
+    if (tile.mode == PASS_TASKS) {
+        ExplicitElemArgs ea{a.ps, a.fem, a.mt, d_partial.get()};
+        if (!launch_explicit_elem(S, Q, ea, eng->stream)) throw std::invalid_argument("unsupported element");
+        launch_explicit_vertex(a.ps, a.own, d_partial.get(), S, n_vertices, _damping, eng->stream);
+        return;
+    }
     if (!launch_gather(S, Q, tile.L, tile.QL, a, eng->stream)) throw std::invalid_argument("unsupported gather tile");
 }
 

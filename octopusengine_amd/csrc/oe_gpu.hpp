@@ -263,11 +263,15 @@ struct GPU_FEM_Explicit final : GPU_FEM {
                      Material material, float damping);
     void step(GPU_ParticleSystem* ps, float dt) override;
     float _damping;
+    // mode PASS_TASKS (the default): element-centric force pass (one element
+    // per 32 lanes, then a per-vertex sum of the element partials);
+    // PASS_GATHER: the per-vertex gather with tile L x QL
     GatherTile tile;
     int n_vertices;
     DBuf<int> d_slot_vid;
     DBuf<int2> d_slot_nbo;
     DBuf<int> d_own;
+    DBuf<float> d_partial;   // E * S * 9: per (element, local node) force and Hessian block
 };
 
 struct GPU_Fix_Constraint final : GPU_Dynamic {

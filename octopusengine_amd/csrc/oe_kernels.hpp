@@ -141,6 +141,8 @@ __device__ __forceinline__ float det_from_cof(const float F[9], const float C[9]
 // The reference assembles H_r from six 3x3 blocks of d2W/dF2; the
 // lambda (J - alpha) d2J/dF2 blocks are antisymmetric and cancel in the
 // symmetric assembly, which leaves the closed form above.
+__device__ __forceinline__ void nh_node(const float F[9], const float C[9], float s, float V, const float g[3],
+                                        const MaterialView& mt, Acc9& a);
 __device__ __forceinline__ void nh_contrib(const float Jx[9], const float JXi[9], float V, const float* __restrict__ dnr,
                                            const MaterialView& mt, Acc9& a) {
     float F[9];
@@ -153,6 +155,14 @@ __device__ __forceinline__ void nh_contrib(const float Jx[9], const float JXi[9]
     cofactor(F, C);
     const float J = det_from_cof(F, C);
     const float s = mt.lambda * (J - mt.alpha);
+    nh_node(F, C, s, V, g, mt, a);
+}
+
+// The part of nh_contrib that depends on node r (through g = JX_inv^T dN_r),
+// given the point's F, cof F and s = lambda (J - alpha): element-centric
+// callers form those once per point and add every node's terms.
+__device__ __forceinline__ void nh_node(const float F[9], const float C[9], float s, float V, const float g[3],
+                                        const MaterialView& mt, Acc9& a) {
     // P g = mu F g + s C g ; c = C g
     float cg[3], Fg[3];
 #pragma unroll

@@ -150,7 +150,10 @@ std::vector<GatherTile> tile_candidates(Element el, bool vbd, bool tasks_ok) {
     std::vector<GatherTile> out;
     for (int L : {4, 8, 16, 32, 64})
         for (int ql : qls) out.push_back(GatherTile{L, ql, PASS_GATHER});
-    if (!vbd) return out;
+    if (!vbd) {   // the explicit force pass: the element-centric form
+        out.push_back(GatherTile{64, 1, PASS_TASKS});
+        return out;
+    }
     for (int L : {4, 8, 16, 32, 64}) out.push_back(GatherTile{L, 1, PASS_CACHE});
     if (!tasks_ok) return out;
     const int maxi = el == Hexa27 ? kQ2FusedIncPerBlock : max_task_inc(el);
@@ -180,7 +183,8 @@ void apply_scene_tile(Scene& sc, const GatherTile& t) {
             throw;
         }
     }
-    if (sc.explicit_fem) sc.explicit_fem->tile = GatherTile{t.L, t.QL, PASS_GATHER};
+    if (sc.explicit_fem)
+        sc.explicit_fem->tile = t.mode == PASS_TASKS ? GatherTile{64, 1, PASS_TASKS} : GatherTile{t.L, t.QL, PASS_GATHER};
     sc.invalidate_graph();
 }
 
@@ -213,7 +217,7 @@ TuneRecord autotune_scene(Scene& sc, bool force) {
     std::vector<std::pair<int, std::vector<bool>>> seen;
     std::vector<GatherTile> unique;
     for (const GatherTile& t : cands) {
-        if (t.mode != PASS_TASKS) {
+        if (t.mode != PASS_TASKS || !vbd) {
             unique.push_back(t);
             continue;
         }

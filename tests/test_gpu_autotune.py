@@ -45,6 +45,28 @@ def test_scene_autotune_then_step_matches_oracle(engine, el, cells):
     assert len(recs) == 1 and recs[0]["candidates"] == t["candidates"]
 
 
+def test_explicit_scene_autotune_includes_the_element_pass(engine):
+    """The explicit force pass tunes over the gather tiles and the
+    element-centric pass (mode 2).  On small beams one gather launch can beat
+    the element pass's two launches, so either may win; the tuned layout is the
+    scene's and the state is put back."""
+    oe.autotune_clear()
+    kw = dict(solver=2, material=2, young=1e6, poisson=0.45, noise=1e-3, seed=2)
+    g, o, _, _ = make_pair(engine, 9, (8, 2, 2), **kw)
+    g.step(1e-3, 10)
+    o.step(1e-3, 10)
+    before = (g.positions().copy(), g.velocities().copy())
+    t = g.autotune()
+    print(f"explicit Q2: tuned {t}", flush=True)
+    assert t["candidates"] > 1 and t["mode"] in (oe.PASS_GATHER, oe.PASS_TASKS)
+    assert g.get_tile_ex()["mode"] == t["mode"]
+    for a, b in zip(before, (g.positions(), g.velocities())):
+        assert np.array_equal(a, b)
+    g.step(1e-3, 10)
+    o.step(1e-3, 10)
+    assert max_err(g.positions(), o.positions()) <= TOL1
+
+
 def test_engine_autotune_on_create(engine):
     oe.autotune_clear()
     eng = oe.Engine(0)

@@ -89,3 +89,31 @@ def test_explicit_damping(engine):
     g.step(1e-3, 100)
     o.step(1e-3, 100)
     assert max_err(g.positions(), o.positions()) <= TOL1
+
+
+FORCE_PASS_CASES = [(9, (4, 2, 2)), (8, (3, 1, 1)), (7, (4, 2, 2)), (6, (6, 2, 2)), (3, (8, 2, 2))]
+
+
+@pytest.mark.parametrize("case", FORCE_PASS_CASES, ids=[f"el{c[0]}" for c in FORCE_PASS_CASES])
+def test_explicit_element_pass_matches_the_gather(engine, case):
+    """The element-centric force pass (the default, mode 2) and the per-vertex
+    gather (mode 0) add the same terms in another order: both follow the oracle
+    within TOL1 and each other within a hundredth of it."""
+    import octopusengine_amd as oe
+
+    el, cells = case
+    kw = dict(solver=2, material=2, young=1e6, poisson=0.45, damping=1e-5, noise=1e-3, seed=5)
+    g, o, rest, _ = make_pair(engine, el, cells, **kw)
+    h, _, _, _ = make_pair(engine, el, cells, **kw)
+    assert g.get_tile_ex()["mode"] == oe.PASS_TASKS
+    h.set_tile(16, 1, oe.PASS_GATHER)
+    assert h.get_tile_ex()["mode"] == oe.PASS_GATHER
+    for s in (g, h, o):
+        s.step(1e-3, 100)
+    err = max_err(g.positions(), o.positions())
+    diff = max_err(g.positions(), h.positions())
+    moved = max_err(o.positions(), rest)
+    print(f"explicit el={el} element pass err={err:.3e} vs gather {diff:.3e} moved={moved:.3e}", flush=True)
+    assert moved > 1e-4
+    assert err <= TOL1
+    assert diff <= 1e-2 * TOL1
