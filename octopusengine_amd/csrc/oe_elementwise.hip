@@ -229,23 +229,32 @@ __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* sl
 }
 
 // Internal force of each vertex from the element partials of k_explicit_elem
-// (9 floats per (element, local node)): the force and Hessian blocks of the
-// vertex's owners entries are added in owners order, then the damping term
-// f = sum(-V P g) - k H v of the reference's force pass
-// (src/GPU/Explicit/GPU_FEM_Explicit.cu:45-62) overwrites ps.f.
-__global__ __launch_bounds__(kBlock) void k_explicit_vertex(PSView ps, OwnersView own, const float* partial, int S,
-                                                            int count, float damping) {
+// (9 floats per (element, local node), in owners order, so a vertex's partials
+// are contiguous): they are added in owners order, then the damping term of
+// the reference's force pass, f = sum(-V P g) - k H v
+// (src/GPU/Explicit/GPU_FEM_Explicit.cu:45-62), overwrites ps.f.  Four
+// incidences are loaded at a time so that their loads overlap.
+constexpr int kExplicitVertexBlock = 64;
+__global__ __launch_bounds__(kExplicitVertexBlock) void k_explicit_vertex(PSView ps, OwnersView own,
+                                                                          const float* partial, int count,
+                                                                          float damping) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
     const int vid = gload(own.slot_vid, slot);
     const int2 nbo = gload(own.slot_nbo, slot);
+    const float* __restrict__ src = partial + static_cast<size_t>(nbo.y) * 9;
     float a[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int j = 0; j < nbo.x; ++j) {
-        const int pk = gload(own.own, nbo.y + j);
-        const size_t at = (static_cast<size_t>(pk >> 5) * S + (pk & 31)) * 9;
+    int j = 0;
+    for (; j + 4 <= nbo.x; j += 4) {
+        float t[36];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) a[i] += gload(partial, at + i);
+        for (int k = 0; k < 36; ++k) t[k] = gload(src, j * 9 + k);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) a[k % 9] += t[k];
     }
+    for (; j < nbo.x; ++j)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[i] += gload(src, j * 9 + i);
     float4 f = make_float4(a[0], a[1], a[2], 0.f);
     if (damping != 0.f) {
         const float4 v = ps.v[vid];
@@ -371,11 +380,11 @@ void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipS
                        make_float3(pmax[0], pmax[1], pmax[2]));
 }
 
-void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int S, int count,
-                            float damping, hipStream_t s) {
+void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int count, float damping,
+                            hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_explicit_vertex, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ps, own, partial,
-                       S, count, damping);
+    hipLaunchKernelGGL(k_explicit_vertex, dim3((count + kExplicitVertexBlock - 1) / kExplicitVertexBlock),
+                       dim3(kExplicitVertexBlock), 0, s, ps, own, partial, count, damping);
 }
 
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,

@@ -25,9 +25,9 @@ void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipS
 void launch_fix_box(const PSView& ps, const unsigned char* fixed, const FixTransform& t, const float pmin[3],
                     const float pmax[3], hipStream_t s);
 // Explicit integrator: each vertex's internal force from the element partials
-// (9 floats per (element, local node)) and the damping term.
-void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int S, int count,
-                            float damping, hipStream_t s);
+// (9 floats per (element, local node), owners order) and the damping term.
+void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int count, float damping,
+                            hipStream_t s);
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,
                         hipStream_t s);
 

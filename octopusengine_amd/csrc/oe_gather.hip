@@ -613,42 +613,60 @@ bool launch_L(int L, const GatherArgs& a, hipStream_t s) {
 
 // Element-centric internal forces of the explicit integrator.  The reference
 // computes them per vertex (kernel_fem_eval_force,
-// src/GPU/Explicit/GPU_FEM_Explicit.cu:5-64), so a vertex rebuilds F at every point
-// of each incident element: S times over per (element, point).  Here each
-// 32-lane half of a wave takes one element:
+// src/GPU/Explicit/GPU_FEM_Explicit.cu:5-64), so a vertex rebuilds F at every
+// point of each incident element: S times over per (element, point).  Here LPE
+// lanes (32, or a whole wave for 27-point elements) take one element:
 //   nodes:  lane k < S puts node k's displacement relative to node 0 in LDS;
 //   points: lane q < Q forms Ju = sum_k x_k (x) dN_k(q), F, cof F and
-//           s = lambda (J - alpha) once and puts them in LDS, point-major per
-//           component (lanes write consecutive words);
-//   nodes:  lane r < S adds the force -V (mu F g + s C g) and Hessian block of
-//           node r over the Q points (g = JX_inv^T dN_r(q); every lane of a
-//           half reads the same point words, a broadcast) and stores the 9
-//           floats at partial[(e S + r) 9].
+//           s = lambda (J - alpha) once, and from them (M = JX_inv^T, so that
+//           g = M dN_r):
+//             Pm = V (mu F + s C) M    force  -V (mu F g + s C g) = -Pm dN_r
+//             G  = V mu M^T M          V mu |g|^2 = dN_r^T G dN_r
+//             K  = C M                 C g = K dN_r
+//           and V lambda, in LDS point-major per component (lanes write
+//           consecutive words);
+//   nodes:  lane r < S of each 32-lane half adds the force and Hessian block
+//           V mu |g|^2 I + V lambda (C g)(C g)^T of node r over its half of
+//           the points (every lane reads the same point words, a broadcast);
+//           the halves are added and lane r stores the 9 floats at the
+//           owners position of (e, r), so that each vertex's partials are
+//           contiguous.
 // k_explicit_vertex then adds each vertex's partials in owners order.
-constexpr int kExplicitHalves = 4;   // elements per block (128 threads)
-constexpr int kExplicitPt = 29;      // F 9, cof F 9, JX_inv 9, s, V
+constexpr int kExplicitThreads = 128;
+constexpr int kExplicitPt = 25;      // Pm 9, K 9, G 6, V lambda
+
+template <int Q>
+constexpr int explicit_lanes() {
+    return Q >= 16 ? 64 : 32;
+}
 
 template <int S, int Q>
-__global__ __launch_bounds__(kExplicitHalves * 32) void k_explicit_elem(ExplicitElemArgs a) {
-    static_assert(S <= 32 && Q <= 32, "one element per 32 lanes");
+__global__ __launch_bounds__(kExplicitThreads) void k_explicit_elem(ExplicitElemArgs a) {
+    static_assert(S <= 32 && Q <= 32, "one element per 32 or 64 lanes");
+    constexpr int LPE = explicit_lanes<Q>();
+    constexpr int EPB = kExplicitThreads / LPE;
+    constexpr int QH = (Q + LPE / 32 - 1) / (LPE / 32);   // points per 32-lane half
     __shared__ float dn_s[Q * S * 3];
-    __shared__ float xs_s[kExplicitHalves][S * 3];
-    __shared__ float pt_s[kExplicitHalves][kExplicitPt][Q];
+    __shared__ float xs_s[EPB][S * 3];
+    __shared__ float pt_s[EPB][kExplicitPt][Q];
     const int tid = threadIdx.x;
-    const int h = tid >> 5, lane = tid & 31;
+    const int h = tid / LPE, lane = tid % LPE;
+    const int r = (lane & 31) < S ? (lane & 31) : S - 1;
+    const int half = lane >> 5;
     const int E = a.fem.E;
-    const int e = blockIdx.x * kExplicitHalves + h;
+    const int e = blockIdx.x * EPB + h;
     const int ec = e < E ? e : E - 1;
-    for (int i = tid; i < Q * S * 3; i += kExplicitHalves * 32) dn_s[i] = gload(a.fem.dN, i);
+    const int at = gload(a.inc_at, static_cast<size_t>(ec) * S + r);
+    for (int i = tid; i < Q * S * 3; i += kExplicitThreads) dn_s[i] = gload(a.fem.dN, i);
     {
         const int k = lane < S ? lane : S - 1;
         const int node = gload(a.fem.topo, static_cast<size_t>(ec) * S + k);
         const float4 u = gload(static_cast<const float4*>(a.ps.p), node);
         float4 ref;
-        ref.x = __shfl(u.x, 0, 32);
-        ref.y = __shfl(u.y, 0, 32);
-        ref.z = __shfl(u.z, 0, 32);
-        ref.w = __shfl(u.w, 0, 32);
+        ref.x = __shfl(u.x, 0, LPE);
+        ref.y = __shfl(u.y, 0, LPE);
+        ref.z = __shfl(u.z, 0, LPE);
+        ref.w = __shfl(u.w, 0, LPE);
         float x[3];
         rel_node(u, ref, a.ps.lo_unit, x);
         if (lane < S) {
@@ -670,49 +688,77 @@ __global__ __launch_bounds__(kExplicitHalves * 32) void k_explicit_elem(Explicit
         for (int k = 0; k < S; ++k) {
             const float d0 = dn[3 * k], d1 = dn[3 * k + 1], d2 = dn[3 * k + 2];
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const float xk = xs_s[h][3 * k + r];
-                Ju[0 + r] = fmaf(xk, d0, Ju[0 + r]);
-                Ju[3 + r] = fmaf(xk, d1, Ju[3 + r]);
-                Ju[6 + r] = fmaf(xk, d2, Ju[6 + r]);
+            for (int c = 0; c < 3; ++c) {
+                const float xk = xs_s[h][3 * k + c];
+                Ju[0 + c] = fmaf(xk, d0, Ju[0 + c]);
+                Ju[3 + c] = fmaf(xk, d1, Ju[3 + c]);
+                Ju[6 + c] = fmaf(xk, d2, Ju[6 + c]);
             }
         }
         float F[9], C[9];
         deformation_gradient(Ju, JXi, F);
         cofactor(F, C);
         const float J = det_from_cof(F, C);
+        const float sl = a.mt.lambda * (J - a.mt.alpha);
         if (lane < Q) {
+            // F, C, JX_inv column-major: X(row, col) = X[col * 3 + row]; M(c, l) = JX_inv[c * 3 + l]
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                pt_s[h][i][lane] = F[i];
-                pt_s[h][9 + i][lane] = C[i];
-                pt_s[h][18 + i][lane] = JXi[i];
-            }
-            pt_s[h][27][lane] = a.mt.lambda * (J - a.mt.alpha);
-            pt_s[h][28][lane] = V;
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    float pm = 0.f, km = 0.f;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        pm += (a.mt.mu * F[c * 3 + i] + sl * C[c * 3 + i]) * JXi[c * 3 + l];
+                        km += C[c * 3 + i] * JXi[c * 3 + l];
+                    }
+                    pt_s[h][i * 3 + l][lane] = V * pm;
+                    pt_s[h][9 + i * 3 + l][lane] = km;
+                }
+            const float vm = V * a.mt.mu;
+            constexpr int gl[6] = {0, 0, 0, 1, 1, 2}, gm[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                pt_s[h][18 + i][lane] = vm * (JXi[gl[i]] * JXi[gm[i]] + JXi[3 + gl[i]] * JXi[3 + gm[i]] +
+                                              JXi[6 + gl[i]] * JXi[6 + gm[i]]);
+            pt_s[h][24][lane] = V * a.mt.lambda;
         }
     }
     __syncthreads();
-    if (e >= E || lane >= S) return;
-    Acc9 acc;
-    acc_zero(acc);
-    for (int p = 0; p < Q; ++p) {
-        float F[9], C[9], Xi[9];
+    float acc[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            F[i] = pt_s[h][i][p];
-            C[i] = pt_s[h][9 + i][p];
-            Xi[i] = pt_s[h][18 + i][p];
+    for (int i = 0; i < 9; ++i) acc[i] = 0.f;
+    const int p_end = (half + 1) * QH < Q ? (half + 1) * QH : Q;
+    for (int p = half * QH; p < p_end; ++p) {
+        const float* __restrict__ dn = dn_s + (p * S + r) * 3;
+        const float d0 = dn[0], d1 = dn[1], d2 = dn[2];
+        float w[kExplicitPt];
+#pragma unroll
+        for (int i = 0; i < kExplicitPt; ++i) w[i] = pt_s[h][i][p];
+        float c[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            acc[i] -= w[i * 3] * d0 + w[i * 3 + 1] * d1 + w[i * 3 + 2] * d2;
+            c[i] = w[9 + i * 3] * d0 + w[9 + i * 3 + 1] * d1 + w[9 + i * 3 + 2] * d2;
         }
-        const float* __restrict__ dn = dn_s + (p * S + lane) * 3;
-        float g[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) g[c] = Xi[c * 3 + 0] * dn[0] + Xi[c * 3 + 1] * dn[1] + Xi[c * 3 + 2] * dn[2];
-        nh_node(F, C, pt_s[h][27][p], pt_s[h][28][p], g, a.mt, acc);
+        const float gd = w[18] * d0 * d0 + w[21] * d1 * d1 + w[23] * d2 * d2 +
+                         2.f * (w[19] * d0 * d1 + w[20] * d0 * d2 + w[22] * d1 * d2);
+        const float vl = w[24];
+        acc[3] += gd + vl * c[0] * c[0];
+        acc[4] += vl * c[1] * c[0];
+        acc[5] += vl * c[2] * c[0];
+        acc[6] += gd + vl * c[1] * c[1];
+        acc[7] += vl * c[2] * c[1];
+        acc[8] += gd + vl * c[2] * c[2];
     }
-    float* out = a.partial + (static_cast<size_t>(e) * S + lane) * 9;
+    if constexpr (LPE == 64) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) out[i] = acc.v[i];
+        for (int i = 0; i < 9; ++i) acc[i] += __shfl_xor(acc[i], 32);
+    }
+    if (e >= E || lane >= S) return;
+    float* out = a.partial + static_cast<size_t>(at) * 9;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = acc[i];
 }
 
 }  // namespace
@@ -808,8 +854,9 @@ void OE_FN(launch_batch_fused_tasks_)(const BatchFusedArgs& b, int nblocks, hipS
 
 void OE_FN(launch_explicit_elem_)(const ExplicitElemArgs& a, hipStream_t s) {
     if (a.fem.E <= 0) return;
-    hipLaunchKernelGGL((k_explicit_elem<OE_GATHER_S, OE_GATHER_Q>),
-                       dim3((a.fem.E + kExplicitHalves - 1) / kExplicitHalves), dim3(kExplicitHalves * 32), 0, s, a);
+    constexpr int EPB = kExplicitThreads / explicit_lanes<OE_GATHER_Q>();
+    hipLaunchKernelGGL((k_explicit_elem<OE_GATHER_S, OE_GATHER_Q>), dim3((a.fem.E + EPB - 1) / EPB),
+                       dim3(kExplicitThreads), 0, s, a);
 }
 
 void OE_FN(launch_batch_ju_refresh_)(const BatchRefreshArgs& b, int nblocks, hipStream_t s) {

@@ -145,12 +145,14 @@ struct BatchRefreshArgs {
 
 // Element-centric internal forces of the explicit integrator: per element and
 // local node r, the 9 floats (force, then the Hessian block xx xy xz yy yz zz)
-// of node r over the element's Q points at partial[(e S + r) 9].
+// of node r over the element's Q points at partial[inc_at[e S + r] 9], where
+// inc_at is the owners position of (e, r).
 struct ExplicitElemArgs {
     PSView ps;
     FemView fem;
     MaterialView mt;
-    float* partial;   // E * S * 9
+    const int* inc_at;   // E * S
+    float* partial;      // E * S * 9, owners order
 };
 
 #define OE_GATHER_DECL(S, Q)                                                                                  \

@@ -587,7 +587,13 @@ GPU_FEM_Explicit::GPU_FEM_Explicit(Engine* e, Element el, const HostMesh& mesh, 
     one.colors.assign(n_vertices, 0);
     const OwnersCSR o = build_owners(mesh, one);   // every vertex, ascending id
     upload_owners(eng, o, d_slot_vid, d_slot_nbo, d_own);
-    d_partial = DBuf<float>(&eng->pool, static_cast<size_t>(mesh.nb_elements()) * S * 9);
+    if (o.own.size() != static_cast<size_t>(mesh.nb_elements()) * S)
+        throw std::invalid_argument("explicit force pass: an element lists a vertex twice");
+    std::vector<int> inc_at(o.own.size(), 0);
+    for (size_t j = 0; j < o.own.size(); ++j)
+        inc_at[static_cast<size_t>(o.own[j] >> 5) * S + (o.own[j] & 31)] = static_cast<int>(j);
+    d_inc_at = DBuf<int>(&eng->pool, inc_at);
+    d_partial = DBuf<float>(&eng->pool, o.own.size() * 9);
 }
 
 void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
@@ -603,9 +609,9 @@ void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
     a.count = n_vertices;
     a.mode = GATHER_EXPLICIT_FORCE;
     if (tile.mode == PASS_TASKS) {
-        ExplicitElemArgs ea{a.ps, a.fem, a.mt, d_partial.get()};
+        ExplicitElemArgs ea{a.ps, a.fem, a.mt, d_inc_at.get(), d_partial.get()};
         if (!launch_explicit_elem(S, Q, ea, eng->stream)) throw std::invalid_argument("unsupported element");
-        launch_explicit_vertex(a.ps, a.own, d_partial.get(), S, n_vertices, _damping, eng->stream);
+        launch_explicit_vertex(a.ps, a.own, d_partial.get(), n_vertices, _damping, eng->stream);
         return;
     }
     if (!launch_gather(S, Q, tile.L, tile.QL, a, eng->stream)) throw std::invalid_argument("unsupported gather tile");

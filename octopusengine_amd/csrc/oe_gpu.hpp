@@ -271,7 +271,8 @@ struct GPU_FEM_Explicit final : GPU_FEM {
     DBuf<int> d_slot_vid;
     DBuf<int2> d_slot_nbo;
     DBuf<int> d_own;
-    DBuf<float> d_partial;   // E * S * 9: per (element, local node) force and Hessian block
+    DBuf<int> d_inc_at;      // E * S: owners position of (element, local node)
+    DBuf<float> d_partial;   // E * S * 9: per (element, local node) force and Hessian block, owners order
 };
 
 struct GPU_Fix_Constraint final : GPU_Dynamic {
