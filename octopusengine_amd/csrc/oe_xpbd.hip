@@ -125,10 +125,18 @@ __global__ __launch_bounds__(256) void k_xpbd_v0(XpbdArgs a) {
     }
 }
 
+// With LPE = 64 (27-node hexahedra) the two 32-lane halves of the wave split
+// the node sum of each point's Jx and the point sum of each node's gradient,
+// and add their parts with one cross-half shuffle, halving the chain of
+// dependent LDS reads and FMAs a wave runs per projection.
 template <int S, int Q, int LPE>
 __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
-    static_assert(S <= LPE && Q <= LPE, "element does not fit the lane group");
+    constexpr int SPLIT = LPE > 32 ? LPE / 32 : 1;
+    constexpr int W = LPE / SPLIT;   // lanes per part
+    static_assert(S <= W && Q <= W, "element does not fit the lane group");
     constexpr int EPB = 256 / LPE;
+    constexpr int KH = (S + SPLIT - 1) / SPLIT;   // nodes per part (point stage)
+    constexpr int QH = (Q + SPLIT - 1) / SPLIT;   // points per part (node stage)
     __shared__ float sx[EPB][S][3];
     __shared__ float sP[EPB][Q][9];
     // Large shape-derivative tables (27-node hexahedra, 8.7 KB) are staged in
@@ -141,6 +149,7 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
     const float* __restrict__ dnt = kDnLds ? static_cast<const float*>(dn_s) : a.fem.dN;
     const int le = threadIdx.x / LPE;
     const int lane = threadIdx.x % LPE;
+    const int sub = lane % W, part = lane / W;
     const int ei = blockIdx.x * EPB + le;
     const bool valid = ei < a.count;
     const int eid = valid ? a.eids[a.first + ei] : 0;
@@ -154,18 +163,23 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
         wv = a.ps.w[vid];
     }
     float JXi[9], V;
-    load_element_record(a.fem, eid, lane < Q ? lane : Q - 1, JXi, V);
+    load_element_record(a.fem, eid, sub < Q ? sub : Q - 1, JXi, V);
     __syncthreads();
+    // constant bounds without a split, so that those loops unroll fully
+    const int k_begin = SPLIT == 1 ? 0 : part * KH;
+    const int k_end = SPLIT == 1 ? S : ((part + 1) * KH < S ? (part + 1) * KH : S);
+    const int q_begin = SPLIT == 1 ? 0 : part * QH;
+    const int q_end = SPLIT == 1 ? Q : ((part + 1) * QH < Q ? (part + 1) * QH : Q);
     for (int m = 0; m < 2; ++m) {
         float e_loc = 0.f;
-        if (valid && lane < Q) {
-            const int q = lane;
+        {
+            const int q = sub < Q ? sub : Q - 1;
             const float* dn = dnt + q * S * 3;
-            float Jx[9], F[9], P[9], Ps[9], e;
+            float Jx[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) Jx[i] = 0.f;
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
+            for (int k = k_begin; k < k_end; ++k) {
                 const float d0 = dn[3 * k], d1 = dn[3 * k + 1], d2 = dn[3 * k + 2];
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
@@ -173,27 +187,39 @@ __global__ __launch_bounds__(256) void k_xpbd_v2(XpbdArgs a) {
                     Jx[r] += xr * d0; Jx[3 + r] += xr * d1; Jx[6 + r] += xr * d2;
                 }
             }
-            deformation_gradient(Jx, JXi, F);
-            xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
-            scale_stress(P, JXi, V, Ps);
-            e_loc = e * V;
+            if constexpr (SPLIT == 2) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) sP[le][q][i] = Ps[i];
+                for (int i = 0; i < 9; ++i) Jx[i] += __shfl_xor(Jx[i], 32);
+            }
+            if (valid && part == 0 && sub < Q) {
+                float F[9], P[9], Ps[9], e;
+                deformation_gradient(Jx, JXi, F);
+                xpbd_material(a.mt.material, m, 1.f + a.mt.mu / a.mt.lambda, F, P, e);
+                scale_stress(P, JXi, V, Ps);
+                e_loc = e * V;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) sP[le][q][i] = Ps[i];
+            }
         }
         const float C = group_sum<LPE>(e_loc);
         __syncthreads();
         float g0 = 0.f, g1 = 0.f, g2 = 0.f, sn = 0.f;
-        if (valid && lane < S) {
-            const int k = lane;
+        {
+            const int k = sub < S ? sub : S - 1;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
+            for (int q = q_begin; q < q_end; ++q) {
                 const float* dn = dnt + (q * S + k) * 3;
                 const float* Ps = sP[le][q];
                 g0 += Ps[0] * dn[0] + Ps[3] * dn[1] + Ps[6] * dn[2];
                 g1 += Ps[1] * dn[0] + Ps[4] * dn[1] + Ps[7] * dn[2];
                 g2 += Ps[2] * dn[0] + Ps[5] * dn[1] + Ps[8] * dn[2];
             }
-            sn = (g0 * g0 + g1 * g1 + g2 * g2) * wv;
+            if constexpr (SPLIT == 2) {
+                g0 += __shfl_xor(g0, 32);
+                g1 += __shfl_xor(g1, 32);
+                g2 += __shfl_xor(g2, 32);
+            }
+            if (valid && lane < S) sn = (g0 * g0 + g1 * g1 + g2 * g2) * wv;
         }
         const float ssum = group_sum<LPE>(sn);
         const float stiff = (m == 0) ? a.mt.lambda : a.mt.mu;
@@ -240,7 +266,7 @@ bool launch_xpbd(int S, int Q, bool v0_form, const XpbdArgs& a, hipStream_t s) {
     if (S == 8 && Q == 8) { launch_v2<8, 8, 8>(a, s); return true; }
     if (S == 10 && Q == 4) { launch_v2<10, 4, 16>(a, s); return true; }
     if (S == 20 && Q == 11) { launch_v2<20, 11, 32>(a, s); return true; }
-    if (S == 27 && Q == 27) { launch_v2<27, 27, 32>(a, s); return true; }
+    if (S == 27 && Q == 27) { launch_v2<27, 27, 64>(a, s); return true; }
     return false;
 }
 
