@@ -233,11 +233,15 @@ __global__ __launch_bounds__(kBlock) void k_copy_colour(PSView ps, const int* sl
 // are contiguous): they are added in owners order, then the damping term of
 // the reference's force pass, f = sum(-V P g) - k H v
 // (src/GPU/Explicit/GPU_FEM_Explicit.cu:45-62), overwrites ps.f.  Four
-// incidences are loaded at a time so that their loads overlap.
+// incidences are loaded at a time so that their loads overlap.  With
+// integrate set (a semi-implicit integrator whose only dynamic is this force
+// pass) the thread then runs k_semi_implicit's update of its vertex, the same
+// operations on the same values, and the separate pass is skipped.
 constexpr int kExplicitVertexBlock = 64;
 __global__ __launch_bounds__(kExplicitVertexBlock) void k_explicit_vertex(PSView ps, OwnersView own,
                                                                           const float* partial, int count,
-                                                                          float damping) {
+                                                                          float damping, int integrate, float dt,
+                                                                          float gx, float gy, float gz) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
     const int vid = gload(own.slot_vid, slot);
@@ -262,7 +266,23 @@ __global__ __launch_bounds__(kExplicitVertexBlock) void k_explicit_vertex(PSView
         f.y -= damping * (a[4] * v.x + a[6] * v.y + a[7] * v.z);
         f.z -= damping * (a[5] * v.x + a[7] * v.y + a[8] * v.z);
     }
-    ps.f[vid] = f;
+    if (!integrate) {
+        ps.f[vid] = f;
+        return;
+    }
+    float4 p = ps.p[vid];
+    float4 v = ps.v[vid];
+    const float w = ps.w[vid];
+    ps.last_p[vid] = p;
+    v.x += (gx + f.x * w) * dt;
+    v.y += (gy + f.y * w) * dt;
+    v.z += (gz + f.z * w) * dt;
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    ps.v[vid] = v;
+    ps.p[vid] = p;
+    ps.f[vid] = f4(0.f, 0.f, 0.f, 0.f);
 }
 
 // ------------------------------------------------------------ batched forms
@@ -381,10 +401,12 @@ void launch_box(const PSView& ps, const float pmin[3], const float pmax[3], hipS
 }
 
 void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int count, float damping,
-                            hipStream_t s) {
+                            const float* gravity, float dt, hipStream_t s) {
     if (count <= 0) return;
+    const float g0 = gravity ? gravity[0] : 0.f, g1 = gravity ? gravity[1] : 0.f, g2 = gravity ? gravity[2] : 0.f;
     hipLaunchKernelGGL(k_explicit_vertex, dim3((count + kExplicitVertexBlock - 1) / kExplicitVertexBlock),
-                       dim3(kExplicitVertexBlock), 0, s, ps, own, partial, count, damping);
+                       dim3(kExplicitVertexBlock), 0, s, ps, own, partial, count, damping, gravity ? 1 : 0, dt, g0,
+                       g1, g2);
 }
 
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,

@@ -26,8 +26,10 @@ void launch_fix_box(const PSView& ps, const unsigned char* fixed, const FixTrans
                     const float pmax[3], hipStream_t s);
 // Explicit integrator: each vertex's internal force from the element partials
 // (9 floats per (element, local node), owners order) and the damping term.
+// With gravity set, the semi-implicit update (launch_semi_implicit) of the
+// vertex follows in the same thread.
 void launch_explicit_vertex(const PSView& ps, const OwnersView& own, const float* partial, int count, float damping,
-                            hipStream_t s);
+                            const float* gravity, float dt, hipStream_t s);
 void launch_copy_colour(const PSView& ps, const int* slot_vid, int first, int count, const float4* p_new,
                         hipStream_t s);
 

@@ -192,6 +192,13 @@ void GPU_Integrator::eval_constraints(GPU_ParticleSystem* ps, float dt) {
 }
 
 void GPU_SemiExplicit::step(GPU_ParticleSystem* ps, float dt) {
+    // A lone element-centric explicit force pass integrates in its own vertex
+    // pass (the same update as launch_semi_implicit, one launch fewer).
+    if (_dynamics.size() == 1 && _dynamics[0]->active)
+        if (auto* fe = dynamic_cast<GPU_FEM_Explicit*>(_dynamics[0].get()); fe && fe->step_integrate(ps, dt, gravity)) {
+            if (apply_constraints) eval_constraints(ps, dt);
+            return;
+        }
     eval_dynamics(ps, dt);
     launch_semi_implicit(ps->get_parameters(), dt, gravity, stream);
     if (apply_constraints) eval_constraints(ps, dt);
@@ -611,10 +618,20 @@ void GPU_FEM_Explicit::step(GPU_ParticleSystem* ps, float dt) {
     if (tile.mode == PASS_TASKS) {
         ExplicitElemArgs ea{a.ps, a.fem, a.mt, d_inc_at.get(), d_partial.get()};
         if (!launch_explicit_elem(S, Q, ea, eng->stream)) throw std::invalid_argument("unsupported element");
-        launch_explicit_vertex(a.ps, a.own, d_partial.get(), n_vertices, _damping, eng->stream);
+        launch_explicit_vertex(a.ps, a.own, d_partial.get(), n_vertices, _damping, nullptr, dt, eng->stream);
         return;
     }
     if (!launch_gather(S, Q, tile.L, tile.QL, a, eng->stream)) throw std::invalid_argument("unsupported gather tile");
+}
+
+bool GPU_FEM_Explicit::step_integrate(GPU_ParticleSystem* ps, float dt, const float gravity[3]) {
+    if (tile.mode != PASS_TASKS || ps->n != n_vertices) return false;
+    const PSView v = ps->get_parameters();
+    ExplicitElemArgs ea{v, fem_view(), mt, d_inc_at.get(), d_partial.get()};
+    if (!launch_explicit_elem(S, Q, ea, eng->stream)) throw std::invalid_argument("unsupported element");
+    launch_explicit_vertex(v, OwnersView{d_slot_vid.get(), d_slot_nbo.get(), d_own.get()}, d_partial.get(), n_vertices,
+                           _damping, gravity, dt, eng->stream);
+    return true;
 }
 
 // ------------------------------------------------------------ constraints

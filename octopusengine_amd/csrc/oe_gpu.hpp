@@ -262,6 +262,9 @@ struct GPU_FEM_Explicit final : GPU_FEM {
     GPU_FEM_Explicit(Engine* eng, Element element, const HostMesh& mesh, float young, float poisson,
                      Material material, float damping);
     void step(GPU_ParticleSystem* ps, float dt) override;
+    // The element-centric pass followed by the semi-implicit update in its
+    // vertex pass; false (nothing launched) in the gather mode.
+    bool step_integrate(GPU_ParticleSystem* ps, float dt, const float gravity[3]);
     float _damping;
     // mode PASS_TASKS (the default): element-centric force pass (one element
     // per 32 lanes, then a per-vertex sum of the element partials);
