@@ -255,7 +255,7 @@ def batched_leg(args, eng) -> dict:
     }
 
 
-def config3_leg(eng) -> dict:
+def xpbd_explicit_leg(eng) -> dict:
     """The high-order explicit and XPBD workload on the GPU: Q2 and P3 beams of
     32 x 8 x 8 cells, Neo-Hookean E = 1e6 Pa, nu = 0.45, +-1e-3 m initial noise,
     x = 0 face fixed; XPBD frames of dt = 0.01 s with 50 substeps (V2 kernels)
@@ -604,7 +604,7 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=2, choices=(1, 2),
                     help="streams of a batch step: 2 runs the Q2 colour passes beside the P2 ones")
     ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
-    ap.add_argument("--config3-only", action="store_true",
+    ap.add_argument("--xpbd-explicit-only", action="store_true",
                     help="run only the explicit / XPBD leg (Q2 and P3 32x8x8) and print its figures (profiling)")
     ap.add_argument("--single-only", action="store_true",
                     help="run only the single-scene leg (P2 64x16x16) and print its figures (profiling)")
@@ -622,8 +622,8 @@ def main():
         print(json.dumps(_finite_or_null({"plain_single_scene_p2_64x16x16": plain_single_leg(args, eng)})), flush=True)
         eng.close()
         return
-    if args.config3_only:
-        print(json.dumps(_finite_or_null({"config3": config3_leg(eng)})), flush=True)
+    if args.xpbd_explicit_only:
+        print(json.dumps(_finite_or_null({"xpbd_explicit_32x8x8": xpbd_explicit_leg(eng)})), flush=True)
         eng.close()
         return
     if args.serve_only:
@@ -689,9 +689,9 @@ def main():
         eng = oe.Engine(0)
         out["plain_single_scene_p2_64x16x16"] = plain_single_leg(args, eng)
         progress(f"single-scene leg: {out['plain_single_scene_p2_64x16x16']['ms_per_step']:.3f} ms per step")
-        out["config3"] = config3_leg(eng)
+        out["xpbd_explicit_32x8x8"] = xpbd_explicit_leg(eng)
         progress("config-3 leg: " + ", ".join(f"{k} {v['us_per_substep']:.1f} us/substep"
-                                              for k, v in out["config3"].items()))
+                                              for k, v in out["xpbd_explicit_32x8x8"].items()))
         eng.close()
     if args.copy_bytes > 0:
         out["copy_calibration"] = copy_calibration(args.copy_bytes)
