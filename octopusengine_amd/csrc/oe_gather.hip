@@ -632,7 +632,7 @@ bool launch_L(int L, const GatherArgs& a, hipStream_t s) {
 //           owners position of (e, r), so that each vertex's partials are
 //           contiguous.
 // k_explicit_vertex then adds each vertex's partials in owners order.
-constexpr int kExplicitThreads = 128;
+constexpr int kExplicitThreads = 256;
 constexpr int kExplicitPt = 25;      // Pm 9, K 9, G 6, V lambda
 
 template <int Q>
